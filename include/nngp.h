@@ -1,0 +1,163 @@
+/*
+ * nngp.h -- C ABI of the MI355X (gfx950) NNGP / Vecchia engine.
+ *
+ * Plain C types only: device pointers are raw pointers to HIP device memory
+ * owned by the caller, sizes are int64_t/size_t, the stream is a hipStream_t
+ * passed as void*.  Every call is asynchronous on that stream unless stated.
+ * No function allocates or frees caller memory; scratch space comes from the
+ * caller through (workspace, workspace_bytes), sized by the *_workspace_bytes
+ * query of the same call.  Functions return an nngp_status (0 = success).
+ *
+ * These entry points stand in for the reference's accelerator plug-in surface,
+ * nngpr/cuda_kernel_accelerators.py (CudaKernelAccelerator / RbfAccelerator) and
+ * the vendor factor/solve calls of its torch/cupy workers; each declaration cites
+ * the reference interface it replaces.
+ */
+#ifndef NNGP_H
+#define NNGP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NNGP_ABI_VERSION 1
+#define NNGP_MAX_DIM 64   /* maximum number of input features */
+#define NNGP_MAX_NN 256   /* maximum neighbours per point */
+
+typedef enum {
+    NNGP_OK = 0,
+    NNGP_ERR_INVALID_ARGUMENT = -1,
+    NNGP_ERR_UNSUPPORTED = -2,           /* caller maps to NotImplementedError */
+    NNGP_ERR_NOT_POSITIVE_DEFINITE = -3, /* caller maps to LinAlgError */
+    NNGP_ERR_WORKSPACE_TOO_SMALL = -4,
+    NNGP_ERR_HIP = -5,
+    NNGP_ERR_KNN_OVERFLOW = -6
+} nngp_status;
+
+typedef enum {
+    NNGP_PRECISION_FP64 = 0,         /* plain FP64 form */
+    NNGP_PRECISION_FP32_REFINED = 1  /* FP32 screening/factor + FP64 refinement */
+} nngp_precision;
+
+/*
+ * Decomposed covariance  k(x_i, x_j) = s * exp(-sum_f (x_if - x_jf)^2 / (2 l_f^2))
+ *                                      + C + w * [i == j (by position)]
+ * i.e. WhiteKernel(w) + ConstantKernel(C) + ConstantKernel(s) * RBF(l), the form
+ * the reference reduces a kernel tree to (cuda_kernel_accelerators.py:520-590).
+ * theta_* give the index of each parameter's log in kernel.theta, or -1 when the
+ * parameter is fixed or absent (cuda_kernel_accelerators.py:592-649); ARD length
+ * scales occupy theta_length_scale .. theta_length_scale + n_length_scale - 1.
+ */
+typedef struct {
+    double noise_level;     /* w */
+    double constant_value;  /* C */
+    double rbf_level;       /* s */
+    int32_t n_length_scale; /* 0: no RBF part, 1: isotropic, d: ARD */
+    int32_t theta_noise_level;
+    int32_t theta_constant_value;
+    int32_t theta_rbf_level;
+    int32_t theta_length_scale;
+    int32_t n_theta;        /* length of kernel.theta */
+    double length_scale[NNGP_MAX_DIM];
+} nngp_kernel;
+
+/* Human-readable text of a status code (static storage). */
+const char* nngp_status_string(int status);
+
+/* Text of the last HIP runtime error seen by this thread's calls (static). */
+const char* nngp_last_error_detail(void);
+
+int nngp_abi_version(void);
+
+/*
+ * k-nearest-neighbour search.
+ * Replaces RbfAccelerator.find_nearest_neighbors + the calc_nn kernel
+ * (cuda_kernel_accelerators.py:196-381, 663-745) and NngprWorker.find_nearest_neighbors
+ * (base_nngpr.py:1613-1686).
+ *   ref (n_ref x d), query (n_query x d): FP64 row-major device arrays.
+ *   scale (host, n_scale doubles): coordinates are divided by it before the
+ *     search (kernel-space search with RBF length scales); n_scale 0 = raw.
+ *   k neighbours per query; has_cut != 0 limits query i to refs [0, i + ref_cut)
+ *     (the reference's ref_cut_index; None <=> has_cut == 0).
+ *   out_idx (n_query x k int32, device): ascending (key, index) order, -1 padded.
+ * Key = FP64 squared distance of scaled coordinates, accumulated with FMA over
+ * features in order; ties keep the lower index.  Both precisions return the
+ * same indices: FP32_REFINED screens in FP32 and re-ranks a candidate margin in
+ * FP64, certifying every row (uncertified rows are redone exactly).
+ */
+size_t nngp_knn_workspace_bytes(int64_t n_ref, int64_t n_query, int d, int k, int precision);
+int nngp_knn(const double* ref, int64_t n_ref, const double* query, int64_t n_query, int d,
+             const double* scale, int n_scale, int k, int has_cut, int64_t ref_cut,
+             int precision, int32_t* out_idx, void* workspace, size_t workspace_bytes,
+             void* stream);
+
+/*
+ * Materialised neighbour covariance blocks (and theta gradients).
+ * Replaces RbfAccelerator.fill_nn_kernel + the fill_nn_kernel kernel
+ * (cuda_kernel_accelerators.py:383-518, 748-864) and NngprWorker.fill_nn_kernel
+ * (base_nngpr.py:1781-1826).
+ *   x (n_x x d FP64), nn (n_points x width int32, -1 = empty slot).
+ *   out_k (n_points x width x width FP64, row-major), out_grad (n_points x width
+ *   x width x n_theta FP64, or NULL).
+ * The fused entry points below never materialise these blocks; this one exists
+ * for the plug-in surface and for tests.
+ */
+int nngp_fill_nn_kernel(const double* x, int64_t n_x, int d, const int32_t* nn,
+                        int64_t n_points, int width, const nngp_kernel* kernel,
+                        double* out_k, double* out_grad, void* stream);
+
+/*
+ * Vecchia log-likelihood (and gradient w.r.t. log-hyperparameters) of training
+ * rows [row0, row1).  Replaces the per-batch body of Nngpr._log_likelihood_worker
+ * (base_nngpr.py:549-614): fill -> +alpha -> Cholesky -> mu, sigma -> l_i, and the
+ * gradient, fused into one pass per point that never stores K or K_gradient.
+ *   x (n x d), y (n x ny) FP64; nn (n x m int32): row i's neighbours among rows
+ *   < i (as returned by nngp_knn with has_cut = 1, ref_cut = 0), -1 padded.
+ *   out_loglik (device, 1 double): sum over rows and outputs.
+ *   out_grad (device, n_theta doubles, or NULL when eval_gradient == 0).
+ *   out_point_loglik (device, row1 - row0 doubles, or NULL).
+ * Sums are reduced in a fixed order: results are bitwise reproducible.
+ */
+size_t nngp_vecchia_workspace_bytes(int64_t n_rows, int m, int d, int ny, int n_theta,
+                                    int precision, int eval_gradient);
+int nngp_vecchia_loglik(const double* x, const double* y, int64_t n, int d, int ny,
+                        const int32_t* nn, int m, int64_t row0, int64_t row1,
+                        const nngp_kernel* kernel, double alpha, int precision,
+                        int eval_gradient, double* out_loglik, double* out_grad,
+                        double* out_point_loglik, void* workspace, size_t workspace_bytes,
+                        void* stream);
+
+/*
+ * Only-train prediction.  Replaces Nngpr._predict_worker (base_nngpr.py:1101-1127):
+ * per query, K over [query, its m train neighbours]; alpha on the train
+ * neighbours only; mean = K_xn K^-1 y_N, std = sqrt(K_00 - K_xn K^-1 K_nx).
+ *   x_train (n_train x d), y_train (n_train x ny), x_query (n_query x d) FP64;
+ *   nn (n_query x m int32) indices into x_train, -1 padded.
+ *   out_mean (n_query x ny), out_std (n_query, or NULL).
+ */
+size_t nngp_predict_workspace_bytes(int64_t n_query, int m, int d, int ny, int precision);
+int nngp_predict(const double* x_train, const double* y_train, int64_t n_train, int d,
+                 int ny, const double* x_query, int64_t n_query, const int32_t* nn, int m,
+                 const nngp_kernel* kernel, double alpha, int precision, double* out_mean,
+                 double* out_std, void* workspace, size_t workspace_bytes, void* stream);
+
+/*
+ * Counters of the last nngp_knn / nngp_vecchia_loglik / nngp_predict call that
+ * used this workspace: rows whose kNN certificate failed and were recomputed
+ * exactly, points re-factored in FP64 after the FP32 pivot check.  Synchronises
+ * the stream.  No reference counterpart (diagnostics of the new precision mode).
+ */
+typedef struct {
+    int64_t knn_rows_recomputed;
+    int64_t points_refactored_fp64;
+} nngp_stats;
+int nngp_read_stats(const void* workspace, nngp_stats* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NNGP_H */

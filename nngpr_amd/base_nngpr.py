@@ -1,0 +1,379 @@
+"""Estimator core: scikit-learn style NNGP regressor on the HIP engine.
+
+Keeps the public surface of the reference's Nngpr (base_nngpr.py:26-1464): the
+constructor parameters and defaults, fit / log_marginal_likelihood / predict, the
+sklearn parameter validation and the L-BFGS-B fit driver.  What runs underneath is
+different: the training set lives in device memory, and every log-likelihood or
+prediction is one kNN launch sequence plus one fused fill/factor/solve launch
+sequence on one MI355X (nngpr_amd.hip_ops), instead of per-batch tensor algebra
+distributed over worker processes.
+
+Out of scope here (raise NotImplementedError): sequential conditioning
+(conditioning_method 'full' / 'full-double-nn'), return_cov, sample_y, and kernels
+outside White + Constant * RBF.
+"""
+import multiprocessing
+from numbers import Integral
+from operator import itemgetter
+
+import numpy as np
+import torch
+from sklearn import base as sk_base
+from sklearn.gaussian_process import GaussianProcessRegressor
+from sklearn.preprocessing._data import _handle_zeros_in_scale
+from sklearn.utils import check_random_state
+from sklearn.utils._param_validation import Interval, StrOptions
+from sklearn.utils.validation import validate_data
+
+from . import _native as nat
+from . import batched_kernels
+from . import hip_ops
+from .array_module import TorchModule
+from .kernel_params import KernelDecomposition
+
+
+def _precisions(allow_downcast_f32):
+    """(kNN precision, fill/factor precision) for an allow_downcast_f32 setting.
+
+    'no' -> FP64 everywhere; 'only-nn' (default) -> FP32-screened kNN (same exact
+    neighbour sets) + FP64 factor; 'yes' -> FP32 factor with FP64 refinement too.
+    """
+    if allow_downcast_f32 == "no":
+        return hip_ops.PRECISION_FP64, hip_ops.PRECISION_FP64
+    if allow_downcast_f32 == "yes":
+        return hip_ops.PRECISION_FP32_REFINED, hip_ops.PRECISION_FP32_REFINED
+    return hip_ops.PRECISION_FP32_REFINED, hip_ops.PRECISION_FP64
+
+
+class HipWorker:
+    """Worker bound to one HIP device, exposing the reference's NngprWorker
+    surface (base_nngpr.py:1558-1867) on top of the C ABI."""
+
+    def __init__(self, device="cuda:0", random_gen_batch_byte_size=None,
+                 nn_large_size_preferred=25000, batch_size=None):
+        dev = torch.device(device)
+        if dev.type != "cuda":
+            raise ValueError("HipWorker needs a HIP device ('cuda' or 'cuda:N'), got %r" % device)
+        if dev.index is None:
+            dev = torch.device("cuda", 0)
+        self.device = dev
+        self.device_type = "cuda"
+        self.xp = TorchModule(dev)
+        self.random_gen_batch_byte_size = random_gen_batch_byte_size
+        self.nn_large_size_preferred = nn_large_size_preferred
+        self.batch_size = batch_size
+        self._dec = KernelDecomposition()
+
+    def __getstate__(self):
+        st = dict(self.__dict__)
+        st["device"] = str(self.device)
+        st["_dec"] = None
+        return st
+
+    def __setstate__(self, st):
+        self.__dict__.update(st)
+        self.device = torch.device(st["device"])
+        self._dec = KernelDecomposition()
+
+    # --- NngprWorker surface -------------------------------------------------
+    def get_array_module(self):
+        return self.xp
+
+    def clear_cache(self):
+        hip_ops.clear_workspaces()
+        if torch.cuda.is_available():
+            torch.cuda.empty_cache()
+
+    def decompose(self, kernel):
+        dec = self._dec.update(kernel)
+        if not dec.accelerated:
+            raise NotImplementedError("Cannot accelerate this kernel: %s" % dec.error)
+        return dec
+
+    def kernel_struct(self, kernel):
+        dec = self.decompose(kernel)
+        return nat.kernel_struct(dec.params, dec.mapping, len(kernel.theta)), dec
+
+    def knn_scale(self, kernel, nn_type):
+        if nn_type != "kernel-space":
+            return None
+        dec = self.decompose(kernel)
+        ls = dec.params.get("length_scale")
+        return None if ls is None else np.asarray(ls, dtype=np.float64).reshape(-1)
+
+    def find_nearest_neighbors(self, ref, query, nn_type, kernel, num_nn, ref_cut_index,
+                               precision=hip_ops.PRECISION_FP32_REFINED):
+        """base_nngpr.py:1613-1686 semantics on the HIP kNN (exact FP64 keys)."""
+        ref = self.xp.from_numpy(ref)
+        query = self.xp.from_numpy(query)
+        return hip_ops.knn(ref, query, self.knn_scale(kernel, nn_type), int(num_nn),
+                           ref_cut=ref_cut_index, precision=precision)
+
+    def fill_nn_kernel(self, x, nn_indices, kernel, eval_gradient=False):
+        """base_nngpr.py:1781-1826 semantics on the HIP fill kernel."""
+        ks, _ = self.kernel_struct(kernel)
+        return hip_ops.fill_nn_kernel(self.xp.from_numpy(x), self.xp.from_numpy(nn_indices), ks,
+                                      eval_gradient=eval_gradient)
+
+    def batched_chofactor(self, x, may_overwrite_x=False):
+        return torch.linalg.cholesky(x, upper=False), True
+
+    def batched_chosolve(self, L, y, lower, may_overwerite_b=False):
+        return torch.cholesky_solve(y, L, upper=not lower)
+
+    def batch_size_for_nn(self, num_nn, itemsize, kernel):
+        if self.batch_size is not None:
+            return self.batch_size
+        return self.custom_batch_size_for_nn(num_nn, itemsize, kernel)
+
+    def custom_batch_size_for_nn(self, num_nn, itemsize, kernel):
+        return int(self.nn_large_size_preferred or 25000)
+
+    def check_random_state(self, seed):
+        if seed is None or isinstance(seed, Integral):
+            g = torch.Generator(device=self.device)
+            if seed is not None:
+                g.manual_seed(int(seed))
+            return g
+        if not isinstance(seed, torch.Generator):
+            raise ValueError("Seed is not a torch.Generator instance or an integer")
+        return seed
+
+    def random_normal(self, random_state, mu, sigma, shape):
+        mu = self.xp.from_numpy(np.asarray(mu)).expand(shape)
+        sigma = self.xp.from_numpy(np.asarray(sigma)).expand(shape)
+        return torch.normal(mu, sigma, generator=random_state)
+
+
+class Nngpr(GaussianProcessRegressor):
+    """Nearest-neighbour GP regressor (Datta 2016 / Vecchia) on the HIP engine.
+
+    Parameters are those of the reference (base_nngpr.py:127-191); see
+    TorchNngpr / NumpyNngpr for the user-facing constructors.
+    """
+
+    _parameter_constraints: dict = {
+        **GaussianProcessRegressor._parameter_constraints,
+        "num_nn": [Interval(Integral, 1, None, closed="left")],
+        "nn_type": [StrOptions({"kernel-space", "input-space"})],
+        "batch_size": [Interval(Integral, 1, None, closed="left")],
+        "allow_downcast_f32": [StrOptions({"yes", "no", "only-nn"})],
+        "distribute_method": [StrOptions({"joblib", "multiprocessing",
+                                          "multiprocessing-heterogenous", "sequential"})],
+        "mp_start_method": [StrOptions(set(multiprocessing.get_all_start_methods()))],
+        "random_gen_batch_byte_size": [Interval(Integral, 1, None, closed="left")],
+    }
+
+    def __init__(self, workers, kernel=None, *, alpha=1e-10, optimizer="fmin_l_bfgs_b",
+                 n_restarts_optimizer=0, normalize_y=False, copy_X_train=True, random_state=None,
+                 num_nn=None, nn_type=None, batch_size=None, allow_downcast_f32=None,
+                 distribute_method=None, mp_start_method=None, random_gen_batch_byte_size=None,
+                 distribute_environ=None):
+        super().__init__(kernel=kernel, alpha=alpha, optimizer=optimizer,
+                         n_restarts_optimizer=n_restarts_optimizer, normalize_y=normalize_y,
+                         copy_X_train=copy_X_train, random_state=random_state)
+        self.workers = workers
+        self.num_nn = 32 if num_nn is None else num_nn
+        self.nn_type = "kernel-space" if nn_type is None else nn_type
+        self.allow_downcast_f32 = "only-nn" if allow_downcast_f32 is None else allow_downcast_f32
+        fork = "fork" in multiprocessing.get_all_start_methods()
+        self.distribute_method = (("multiprocessing" if fork else "joblib")
+                                  if distribute_method is None else distribute_method)
+        self.mp_start_method = (("fork" if fork else "spawn")
+                                if mp_start_method is None else mp_start_method)
+        self.random_gen_batch_byte_size = (1024 ** 3 if random_gen_batch_byte_size is None
+                                           else random_gen_batch_byte_size)
+        self.distribute_environ = distribute_environ
+        if self.distribute_environ is None:
+            self.distribute_environ = {"OPENBLAS_NUM_THREADS": "1", "MKL_NUM_THREADS": "1",
+                                       "OMP_NUM_THREADS": "1"}
+        self.batch_size = batch_size
+        if self.batch_size is None:
+            sizes = [w.nn_large_size_preferred for w in self.workers
+                     if w.nn_large_size_preferred is not None]
+            self.batch_size = max(1, min(sizes)) if sizes else 1000
+
+    # ------------------------------------------------------------------ helpers
+    @property
+    def _worker(self):
+        return self.workers[0]
+
+    def get_default_kernel(self):
+        """Fixed C(1) * RBF(1) (base_nngpr.py:1137-1151)."""
+        return batched_kernels.ConstantKernel(1.0, constant_value_bounds="fixed") * \
+            batched_kernels.RBF(1.0, length_scale_bounds="fixed")
+
+    def _device_train(self):
+        """Training arrays resident on the device (uploaded once per fit)."""
+        key = (id(self.X_train_), id(self.y_train_))
+        cache = getattr(self, "_dev_cache", None)
+        if cache is None or cache[0] != key:
+            xp = self._worker.get_array_module()
+            x = xp.from_numpy(np.ascontiguousarray(self.X_train_, dtype=np.float64))
+            y = self.y_train_
+            if y.ndim == 1:
+                y = y[:, None]
+            y = xp.from_numpy(np.ascontiguousarray(y, dtype=np.float64))
+            self._dev_cache = (key, x, y, {})
+        return self._dev_cache[1], self._dev_cache[2], self._dev_cache[3]
+
+    def _train_knn(self, kernel, x_dev, knn_cache, precision):
+        """Vecchia conditioning sets: each row's nearest earlier rows
+        (base_nngpr.py:530-546, ref = x[:i1], query = x[i0:i1], ref_cut = i0)."""
+        n = x_dev.shape[0]
+        k = max(1, min(self.num_nn, n - 1))
+        scale = self._worker.knn_scale(kernel, self.nn_type)
+        key = (None if scale is None else tuple(scale.tolist()), k, self.nn_type, precision)
+        nn = knn_cache.get(key)
+        if nn is None:
+            knn_cache.clear()
+            nn = hip_ops.knn(x_dev, x_dev, scale, k, ref_cut=0, precision=precision)
+            knn_cache[key] = nn
+        return nn
+
+    # ---------------------------------------------------------------------- fit
+    @sk_base._fit_context(prefer_skip_nested_validation=True)
+    def fit(self, X, y):
+        """Fit the regressor (base_nngpr.py:271-383): validation, normalize_y,
+        L-BFGS-B over log-hyperparameters via scikit-learn's optimizer driver."""
+        self.kernel_ = self.get_default_kernel() if self.kernel is None else sk_base.clone(self.kernel)
+        self._rng = check_random_state(self.random_state)
+        if self.kernel_.requires_vector_input:
+            dtype, ensure_2d = "numeric", True
+        else:
+            dtype, ensure_2d = None, False
+        X, y = validate_data(self, X, y, multi_output=True, y_numeric=True, ensure_2d=ensure_2d,
+                             dtype=dtype)
+        n_targets_seen = y.shape[1] if y.ndim > 1 else 1
+        if self.n_targets is not None and n_targets_seen != self.n_targets:
+            raise ValueError("The number of targets seen in `y` is different from the parameter "
+                             "`n_targets`. Got %d != %d." % (n_targets_seen, self.n_targets))
+        if self.normalize_y:
+            self._y_train_mean = np.mean(y, axis=0)
+            self._y_train_std = _handle_zeros_in_scale(np.std(y, axis=0), copy=False)
+            y = (y - self._y_train_mean) / self._y_train_std
+        else:
+            shape = (y.shape[1],) if y.ndim == 2 else 1
+            self._y_train_mean = np.zeros(shape=shape)
+            self._y_train_std = np.ones(shape=shape)
+        if np.iterable(self.alpha):
+            if self.alpha.shape[0] == 1:
+                self.alpha = self.alpha[0]
+            else:
+                raise ValueError("alpha must be a scalar or an array with only one element")
+        self.X_train_ = np.copy(X) if self.copy_X_train else X
+        self.y_train_ = np.copy(y) if self.copy_X_train else y
+
+        if self.optimizer is not None and self.kernel_.n_dims > 0:
+            def obj_func(theta, eval_gradient=True):
+                if eval_gradient:
+                    lml, grad = self.log_marginal_likelihood(theta, eval_gradient=True,
+                                                             clone_kernel=False)
+                    return -lml, -grad
+                return -self.log_marginal_likelihood(theta, clone_kernel=False)
+
+            optima = [self._constrained_optimization(obj_func, self.kernel_.theta,
+                                                     self.kernel_.bounds)]
+            if self.n_restarts_optimizer > 0:
+                if not np.isfinite(self.kernel_.bounds).all():
+                    raise ValueError("Multiple optimizer restarts (n_restarts_optimizer>0) "
+                                     "requires that all bounds are finite.")
+                bounds = self.kernel_.bounds
+                for _ in range(self.n_restarts_optimizer):
+                    theta0 = self._rng.uniform(bounds[:, 0], bounds[:, 1])
+                    optima.append(self._constrained_optimization(obj_func, theta0, bounds))
+            lml_values = list(map(itemgetter(1), optima))
+            self.kernel_.theta = optima[np.argmin(lml_values)][0]
+            self.kernel_._check_bounds_params()
+            self.log_marginal_likelihood_value_ = -np.min(lml_values)
+        else:
+            self.log_marginal_likelihood_value_ = self.log_marginal_likelihood(
+                self.kernel_.theta, clone_kernel=False)
+        for w in self.workers:
+            w.clear_cache()
+        return self
+
+    # ------------------------------------------------------------ log-likelihood
+    def log_marginal_likelihood(self, theta=None, eval_gradient=False, clone_kernel=True):
+        """Vecchia log-likelihood at theta (and its gradient), base_nngpr.py:385-451.
+
+        One kNN pass (exact neighbour sets; cached while the search scale is
+        unchanged) and one fused fill -> Cholesky -> mu/sigma -> loglik (+ adjoint
+        gradient) pass over all training rows, reduced on the device.
+        """
+        if theta is None:
+            if eval_gradient:
+                raise ValueError("Gradient can only be evaluated for theta!=None")
+            return self.log_marginal_likelihood_value_
+        if clone_kernel:
+            kernel = self.kernel_.clone_with_theta(theta)
+        else:
+            kernel = self.kernel_
+            kernel.theta = theta
+        x_dev, y_dev, cache = self._device_train()
+        knn_prec, fac_prec = _precisions(self.allow_downcast_f32)
+        ks, _ = self._worker.kernel_struct(kernel)
+        nn = self._train_knn(kernel, x_dev, cache, knn_prec)
+        ll, grad, _ = hip_ops.vecchia_loglik(x_dev, y_dev, nn, ks, float(self.alpha),
+                                             precision=fac_prec, eval_gradient=eval_gradient)
+        if eval_gradient:
+            g = grad.cpu().numpy().astype(np.float64)
+            return float(ll.item()), g
+        return float(ll.item())
+
+    # ------------------------------------------------------------------ predict
+    def predict(self, X, return_std=False, return_cov=False, conditioning_method=None):
+        """Only-train predictive mean (and std) of X (base_nngpr.py:763-894)."""
+        conditioning_method = "only-train" if conditioning_method is None else conditioning_method
+        if conditioning_method not in {"only-train", "full", "full-double-nn"}:
+            raise ValueError("conditioning_method must be 'only-train', 'full' or 'full-double-nn'")
+        if return_std and return_cov:
+            raise RuntimeError("At most one of return_std or return_cov can be requested.")
+        if self.kernel is None or self.kernel.requires_vector_input:
+            dtype, ensure_2d = "numeric", True
+        else:
+            dtype, ensure_2d = None, False
+        X = validate_data(self, X, ensure_2d=ensure_2d, dtype=dtype, reset=False)
+        if return_cov or conditioning_method != "only-train":
+            raise NotImplementedError(
+                "Sequential conditioning ('full', 'full-double-nn') and return_cov are not "
+                "implemented on the HIP engine; use conditioning_method='only-train'.")
+        worker = self._worker
+        xp = worker.get_array_module()
+        if not hasattr(self, "X_train_"):
+            kernel = self.kernel if self.kernel is not None else self.get_default_kernel()
+            mean = np.zeros(X.shape[0])
+            if return_std:
+                if isinstance(kernel, batched_kernels.BatchedKernel):
+                    diag = xp.to_numpy(kernel.diag(xp, xp.from_numpy(np.asarray(X, dtype=np.float64))))
+                else:
+                    diag = kernel.diag(X)
+                return mean, np.sqrt(diag)
+            return mean
+        kernel = self.kernel_
+        x_dev, y_dev, _ = self._device_train()
+        knn_prec, fac_prec = _precisions(self.allow_downcast_f32)
+        ks, _ = worker.kernel_struct(kernel)
+        xq = xp.from_numpy(np.ascontiguousarray(X, dtype=np.float64))
+        nt = x_dev.shape[0]
+        k = max(1, min(nt, self.num_nn))
+        nn = hip_ops.knn(x_dev, xq, worker.knn_scale(kernel, self.nn_type), k, ref_cut=None,
+                         precision=knn_prec)
+        mean_d, std_d = hip_ops.predict(x_dev, y_dev, xq, nn, ks, float(self.alpha),
+                                        precision=fac_prec, return_std=return_std)
+        mean = mean_d.cpu().numpy()
+        ystd = np.asarray(self._y_train_std).astype(mean.dtype).reshape((1, -1))
+        ymean = np.asarray(self._y_train_mean).astype(mean.dtype).reshape((1, -1))
+        mean = ystd * mean + ymean
+        if mean.shape[1] == 1:
+            mean = mean[:, 0]
+        if not return_std:
+            return mean
+        std = std_d.cpu().numpy()
+        std = std.reshape((-1, 1)) * ystd
+        if std.shape[1] == 1:
+            std = std[:, 0]
+        return mean, std
+
+    def sample_y(self, X, n_samples=1, random_state=0, conditioning_method=None):
+        raise NotImplementedError("sample_y is not implemented on the HIP engine")

@@ -1,0 +1,253 @@
+// extern "C" entry points of libnngp_hip.so (declared in include/nngp.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "common.h"
+#include "knn.h"
+#include "vecchia.h"
+
+namespace nngp {
+
+static thread_local char g_last_error[256] = "";
+
+void set_last_hip_error(hipError_t e, const char* what, int line) {
+    std::snprintf(g_last_error, sizeof(g_last_error), "%s (%d) at capi line %d: %s",
+                  hipGetErrorString(e), (int)e, line, what);
+}
+
+namespace {
+
+int to_kern_params(const nngp_kernel* k, int d, KernParams* out) {
+    if (!k) return NNGP_ERR_INVALID_ARGUMENT;
+    if (k->n_length_scale != 0 && k->n_length_scale != 1 && k->n_length_scale != d)
+        return NNGP_ERR_INVALID_ARGUMENT;
+    if (k->n_theta < 0 || k->n_theta > NNGP_MAX_DIM + 3) return NNGP_ERR_INVALID_ARGUMENT;
+    auto in_range = [&](int t, int span) { return t < 0 || (t + span <= k->n_theta); };
+    if (!in_range(k->theta_noise_level, 1) || !in_range(k->theta_constant_value, 1) ||
+        !in_range(k->theta_rbf_level, 1) ||
+        !in_range(k->theta_length_scale, k->n_length_scale > 0 ? k->n_length_scale : 1))
+        return NNGP_ERR_INVALID_ARGUMENT;
+    KernParams kp{};
+    kp.w = k->noise_level;
+    kp.C = k->constant_value;
+    kp.s = k->n_length_scale > 0 ? k->rbf_level : 0.0;
+    kp.n_ls = k->n_length_scale;
+    kp.t_w = k->theta_noise_level;
+    kp.t_C = k->theta_constant_value;
+    kp.t_s = k->n_length_scale > 0 ? k->theta_rbf_level : -1;
+    kp.t_l = k->n_length_scale > 0 ? k->theta_length_scale : -1;
+    kp.n_theta = k->n_theta;
+    for (int f = 0; f < kp.n_ls; ++f) {
+        const double l = k->length_scale[f];
+        if (!(l > 0.0)) return NNGP_ERR_INVALID_ARGUMENT;
+        kp.h[f] = 1.0 / (2.0 * l * l);
+    }
+    *out = kp;
+    return NNGP_OK;
+}
+
+struct VecLayout {
+    WsHeader* hdr;
+    double* pt;
+    double* partial;
+    double* wsL;
+    double* wsv;
+    int64_t P;
+};
+
+VecLayout carve_vec(Carver& c, int64_t npts, int m, int n_theta, bool need_pt) {
+    VecLayout L{};
+    L.hdr = c.take<WsHeader>(1);
+    const int cols = 1 + n_theta;
+    if (need_pt) {
+        L.pt = c.take<double>((size_t)npts * cols + 1);
+        L.partial = c.take<double>(reduce_partial_doubles(cols));
+    }
+    L.P = generic_chunk(npts, m);
+    L.wsL = c.take<double>((size_t)L.P * ((size_t)m * (m + 1) / 2));
+    L.wsv = c.take<double>((size_t)L.P * 5 * (size_t)m);
+    return L;
+}
+
+__global__ void extract_col_kernel(const double* pt, int stride, int64_t n, double* out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = pt[i * stride];
+}
+
+}  // namespace
+}  // namespace nngp
+
+using namespace nngp;
+
+extern "C" {
+
+const char* nngp_status_string(int status) {
+    switch (status) {
+        case NNGP_OK: return "success";
+        case NNGP_ERR_INVALID_ARGUMENT: return "invalid argument";
+        case NNGP_ERR_UNSUPPORTED: return "unsupported configuration";
+        case NNGP_ERR_NOT_POSITIVE_DEFINITE: return "matrix is not positive definite";
+        case NNGP_ERR_WORKSPACE_TOO_SMALL: return "workspace too small";
+        case NNGP_ERR_HIP: return "HIP runtime error";
+        case NNGP_ERR_KNN_OVERFLOW: return "kNN candidate overflow";
+        default: return "unknown status";
+    }
+}
+
+const char* nngp_last_error_detail(void) { return g_last_error; }
+
+int nngp_abi_version(void) { return NNGP_ABI_VERSION; }
+
+size_t nngp_knn_workspace_bytes(int64_t n_ref, int64_t n_query, int d, int k, int precision) {
+    // Separate query buffers are always budgeted so one size serves both forms.
+    return knn_workspace_bytes(n_ref, n_query, d, k, precision, false);
+}
+
+int nngp_knn(const double* ref, int64_t n_ref, const double* query, int64_t n_query, int d,
+             const double* scale, int n_scale, int k, int has_cut, int64_t ref_cut,
+             int precision, int32_t* out_idx, void* workspace, size_t workspace_bytes,
+             void* stream) {
+    if ((n_ref > 0 && !ref) || (n_query > 0 && (!query || !out_idx)) || (n_scale > 0 && !scale))
+        return NNGP_ERR_INVALID_ARGUMENT;
+    return knn_run(ref, n_ref, query, n_query, d, scale, n_scale, k, has_cut, ref_cut, precision,
+                   out_idx, workspace, workspace_bytes, (hipStream_t)stream, true);
+}
+
+int nngp_fill_nn_kernel(const double* x, int64_t n_x, int d, const int32_t* nn,
+                        int64_t n_points, int width, const nngp_kernel* kernel, double* out_k,
+                        double* out_grad, void* stream) {
+    (void)n_x;
+    if (d < 1 || d > NNGP_MAX_DIM || width < 1 || n_points < 0 || !out_k || !nn || !x)
+        return NNGP_ERR_INVALID_ARGUMENT;
+    KernParams kp;
+    int rc = to_kern_params(kernel, d, &kp);
+    if (rc) return rc;
+    launch_fill(x, d, nn, n_points, width, kp, out_k, out_grad, (hipStream_t)stream);
+    NNGP_HIP_CHECK(hipGetLastError());
+    return NNGP_OK;
+}
+
+size_t nngp_vecchia_workspace_bytes(int64_t n_rows, int m, int d, int ny, int n_theta,
+                                    int precision, int eval_gradient) {
+    (void)d;
+    (void)ny;
+    (void)precision;
+    (void)eval_gradient;
+    Carver c(nullptr, 0);
+    carve_vec(c, n_rows, m, n_theta, true);
+    return c.used + 256;
+}
+
+int nngp_vecchia_loglik(const double* x, const double* y, int64_t n, int d, int ny,
+                        const int32_t* nn, int m, int64_t row0, int64_t row1,
+                        const nngp_kernel* kernel, double alpha, int precision,
+                        int eval_gradient, double* out_loglik, double* out_grad,
+                        double* out_point_loglik, void* workspace, size_t workspace_bytes,
+                        void* stream) {
+    hipStream_t st = (hipStream_t)stream;
+    if (d < 1 || d > NNGP_MAX_DIM || ny < 1 || m < 1 || m > NNGP_MAX_NN || row0 < 0 ||
+        row1 < row0 || row1 > n || !out_loglik)
+        return NNGP_ERR_INVALID_ARGUMENT;
+    if (precision != NNGP_PRECISION_FP64 && precision != NNGP_PRECISION_FP32_REFINED)
+        return NNGP_ERR_INVALID_ARGUMENT;
+    KernParams kp;
+    int rc = to_kern_params(kernel, d, &kp);
+    if (rc) return rc;
+    if (eval_gradient && kp.n_theta > 0 && !out_grad) return NNGP_ERR_INVALID_ARGUMENT;
+    const int64_t npts = row1 - row0;
+    Carver c(workspace, workspace_bytes);
+    VecLayout L = carve_vec(c, npts, m, kp.n_theta, true);
+    if (!c.ok() || !workspace) return NNGP_ERR_WORKSPACE_TOO_SMALL;
+    hipLaunchKernelGGL(zero_header_kernel, dim3(1), dim3(64), 0, st, L.hdr);
+    VecArgs a{};
+    a.x = x;
+    a.y = y;
+    a.xs = x + row0 * d;
+    a.ys = y + row0 * ny;
+    a.nn = nn + row0 * m;
+    a.npts = npts;
+    a.d = d;
+    a.ny = ny;
+    a.m = m;
+    a.alpha = alpha;
+    a.alpha_self = 1;
+    a.kp = kp;
+    a.n_theta = kp.n_theta;
+    a.grad = eval_gradient ? 1 : 0;
+    a.pt = L.pt;
+    a.pt_stride = 1 + kp.n_theta;
+    a.hdr = L.hdr;
+    if (npts > 0) launch_generic(a, L.wsL, L.wsv, L.P, st);
+    launch_reduce(L.pt, npts, 1 + kp.n_theta, L.partial, out_loglik,
+                  eval_gradient ? out_grad : nullptr, st);
+    if (out_point_loglik && npts > 0)
+        hipLaunchKernelGGL(extract_col_kernel, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0,
+                           st, L.pt, 1 + kp.n_theta, npts, out_point_loglik);
+    NNGP_HIP_CHECK(hipGetLastError());
+    return NNGP_OK;
+}
+
+size_t nngp_predict_workspace_bytes(int64_t n_query, int m, int d, int ny, int precision) {
+    (void)d;
+    (void)ny;
+    (void)precision;
+    Carver c(nullptr, 0);
+    carve_vec(c, n_query, m, 0, false);
+    return c.used + 256;
+}
+
+int nngp_predict(const double* x_train, const double* y_train, int64_t n_train, int d, int ny,
+                 const double* x_query, int64_t n_query, const int32_t* nn, int m,
+                 const nngp_kernel* kernel, double alpha, int precision, double* out_mean,
+                 double* out_std, void* workspace, size_t workspace_bytes, void* stream) {
+    hipStream_t st = (hipStream_t)stream;
+    (void)n_train;
+    if (d < 1 || d > NNGP_MAX_DIM || ny < 1 || m < 1 || m > NNGP_MAX_NN || n_query < 0 ||
+        !out_mean)
+        return NNGP_ERR_INVALID_ARGUMENT;
+    if (precision != NNGP_PRECISION_FP64 && precision != NNGP_PRECISION_FP32_REFINED)
+        return NNGP_ERR_INVALID_ARGUMENT;
+    KernParams kp;
+    int rc = to_kern_params(kernel, d, &kp);
+    if (rc) return rc;
+    Carver c(workspace, workspace_bytes);
+    VecLayout L = carve_vec(c, n_query, m, 0, false);
+    if (!c.ok() || !workspace) return NNGP_ERR_WORKSPACE_TOO_SMALL;
+    hipLaunchKernelGGL(zero_header_kernel, dim3(1), dim3(64), 0, st, L.hdr);
+    VecArgs a{};
+    a.x = x_train;
+    a.y = y_train;
+    a.xs = x_query;
+    a.ys = nullptr;
+    a.nn = nn;
+    a.npts = n_query;
+    a.d = d;
+    a.ny = ny;
+    a.m = m;
+    a.alpha = alpha;
+    a.alpha_self = 0;
+    a.kp = kp;
+    a.n_theta = 0;
+    a.grad = 0;
+    a.mean = out_mean;
+    a.stdv = out_std;
+    a.hdr = L.hdr;
+    if (n_query > 0) launch_generic(a, L.wsL, L.wsv, L.P, st);
+    NNGP_HIP_CHECK(hipGetLastError());
+    return NNGP_OK;
+}
+
+int nngp_read_stats(const void* workspace, nngp_stats* out, void* stream) {
+    if (!workspace || !out) return NNGP_ERR_INVALID_ARGUMENT;
+    WsHeader h;
+    NNGP_HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
+    NNGP_HIP_CHECK(hipMemcpy(&h, workspace, sizeof(h), hipMemcpyDeviceToHost));
+    out->knn_rows_recomputed = h.knn_rows_recomputed;
+    out->points_refactored_fp64 = h.points_refactored_fp64;
+    return h.not_pd > 0 ? NNGP_ERR_NOT_POSITIVE_DEFINITE : NNGP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,71 @@
+// Shared device helpers for the NNGP HIP kernels (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/nngp.h"
+
+#define NNGP_WAVE 64
+
+namespace nngp {
+
+// Error-checking helper for host code: returns NNGP_ERR_HIP from the enclosing
+// function on any HIP runtime error.
+#define NNGP_HIP_CHECK(expr)                                 \
+    do {                                                     \
+        hipError_t _e = (expr);                              \
+        if (_e != hipSuccess) {                              \
+            nngp::set_last_hip_error(_e, #expr, __LINE__);   \
+            return NNGP_ERR_HIP;                             \
+        }                                                    \
+    } while (0)
+
+void set_last_hip_error(hipError_t e, const char* what, int line);
+
+template <typename T>
+__device__ __forceinline__ T dev_inf();
+template <>
+__device__ __forceinline__ float dev_inf<float>() { return __builtin_inff(); }
+template <>
+__device__ __forceinline__ double dev_inf<double>() { return __builtin_inf(); }
+
+// Sum over the lanes [base, base + width) of a wave; every lane of the group gets
+// the total.  width is a power of two <= 64.
+template <int WIDTH>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+    for (int off = WIDTH / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, WIDTH);
+    return v;
+}
+
+// Header at offset 0 of every workspace: counters of the last call.
+struct WsHeader {
+    int64_t knn_rows_recomputed;
+    int64_t points_refactored_fp64;
+    int64_t not_pd;
+    int64_t reserved[29];
+};
+
+static __global__ void zero_header_kernel(WsHeader* h) {
+    int i = threadIdx.x;
+    if (i < (int)(sizeof(WsHeader) / sizeof(int64_t))) reinterpret_cast<int64_t*>(h)[i] = 0;
+}
+
+// Workspace carving: bump allocator over a caller-provided device buffer.
+struct Carver {
+    char* base;
+    size_t cap;
+    size_t used;
+    __host__ Carver(void* b, size_t c) : base(static_cast<char*>(b)), cap(c), used(0) {}
+    template <typename T>
+    __host__ T* take(size_t count) {
+        size_t off = (used + 255) & ~size_t(255);
+        used = off + count * sizeof(T);
+        if (base == nullptr) return nullptr;
+        return reinterpret_cast<T*>(base + off);
+    }
+    __host__ bool ok() const { return used <= cap; }
+};
+
+}  // namespace nngp

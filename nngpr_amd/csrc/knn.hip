@@ -1,0 +1,750 @@
+// Brute-force k-nearest-neighbour search for the NNGP engine (gfx950).
+//
+// Key of a (query, ref) pair: FP64 squared distance of the scaled coordinates,
+// key = fma(t_{d-1}, t_{d-1}, ... fma(t_1, t_1, t_0 * t_0)), t_f = u_ref,f - u_query,f;
+// order = ascending (key, ref index).  This is the reference's calc_nn ordering
+// (cuda_kernel_accelerators.py:711-742) evaluated in FP64.
+//
+// Two search forms produce identical indices:
+//   * exact:  one lane per query streams the refs (wave-uniform, read through the
+//             scalar cache), keeps a sorted top-K in VGPRs and an append buffer in
+//             LDS; FP64 keys throughout.
+//   * screen: the same streaming loop in FP32 on centred coordinates with the
+//             expansion |r|^2 - 2 q.r (3 FMAs + 1 compare per pair for d = 3),
+//             keeping K = m + margin candidates; a re-rank pass computes the exact
+//             FP64 keys of the candidates, takes the top m and certifies the row
+//             with a rigorous bound on the FP32 error.  Rows that fail the
+//             certificate are recomputed exactly by the per-row fallback kernel.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "common.h"
+#include "knn.h"
+
+namespace nngp {
+
+namespace {
+
+constexpr int kFallbackThreads = 16384;
+
+struct PrepArgs {
+    const double* x;   // n x d row-major
+    int64_t n;
+    int d;
+    int n_scale;
+    double scale[NNGP_MAX_DIM];
+};
+
+// Monotone map of a double onto int64 so that atomicMin/atomicMax order like the
+// doubles (NaN excluded).
+__device__ __forceinline__ long long dbl_key(double v) {
+    long long b = __double_as_longlong(v);
+    return b >= 0 ? b : (b ^ 0x7fffffffffffffffLL);
+}
+__device__ __forceinline__ double key_dbl(long long k) {
+    long long b = k >= 0 ? k : (k ^ 0x7fffffffffffffffLL);
+    return __longlong_as_double(b);
+}
+
+__device__ __forceinline__ double scaled(const PrepArgs& a, int64_t i, int f) {
+    double v = a.x[i * a.d + f];
+    if (a.n_scale == 1) v = v / a.scale[0];
+    else if (a.n_scale > 1) v = v / a.scale[f];
+    return v;
+}
+
+__global__ void init_minmax_kernel(long long* mm, int d, unsigned int* smax) {
+    int f = threadIdx.x;
+    if (f < d) {
+        mm[2 * f] = 0x7fffffffffffffffLL;
+        mm[2 * f + 1] = (long long)0x8000000000000000ULL;
+    }
+    if (f == 0) *smax = 0u;
+}
+
+__global__ void minmax_kernel(PrepArgs a, long long* mm) {
+    // one thread per point, block-level pre-reduction through LDS
+    __shared__ long long smin[256], smaxv[256];
+    for (int f = 0; f < a.d; ++f) {
+        long long lo = 0x7fffffffffffffffLL, hi = (long long)0x8000000000000000ULL;
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n;
+             i += (int64_t)gridDim.x * blockDim.x) {
+            double v = scaled(a, i, f);
+            if (v == v) {
+                long long k = dbl_key(v);
+                lo = k < lo ? k : lo;
+                hi = k > hi ? k : hi;
+            }
+        }
+        smin[threadIdx.x] = lo;
+        smaxv[threadIdx.x] = hi;
+        __syncthreads();
+        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+            if (threadIdx.x < s) {
+                smin[threadIdx.x] = min(smin[threadIdx.x], smin[threadIdx.x + s]);
+                smaxv[threadIdx.x] = max(smaxv[threadIdx.x], smaxv[threadIdx.x + s]);
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            atomicMin(&mm[2 * f], smin[0]);
+            atomicMax(&mm[2 * f + 1], smaxv[0]);
+        }
+        __syncthreads();
+    }
+}
+
+// Writes, for points [0, n):
+//   u64  (n x d AoS): scaled FP64 coordinates (u = x / scale)
+//   s64  (d x npad SoA): same values, feature-major (exact scan); pad = +inf
+//   s32  ((d+1) x npad SoA, optional): fl32(u - centre) and, in row d, -|fl32(u - c)|^2
+//   p2   (n, optional): |u - centre|^2 in FP64
+//   smax (optional): max_i |u_i - centre| over these points (atomic, float bits)
+//   rows d .. dp-1 of s64/s32 are zero (padding features), the norm row is row dp.
+__global__ void prep_kernel(PrepArgs a, const long long* mm, int64_t npad, int dp, double* u64,
+                            double* s64, float* s32, double* p2, unsigned int* smax) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npad) return;
+    if (i >= a.n) {
+        for (int f = 0; f < dp; ++f) {
+            if (s64) s64[f * npad + i] = f == 0 ? __builtin_inf() : 0.0;
+            if (s32) s32[f * npad + i] = 0.0f;
+        }
+        if (s32) s32[dp * npad + i] = -__builtin_inff();
+        return;
+    }
+    for (int f = a.d; f < dp; ++f) {
+        if (s64) s64[f * npad + i] = 0.0;
+        if (s32) s32[f * npad + i] = 0.0f;
+    }
+    double nrm = 0.0, pp = 0.0;
+    for (int f = 0; f < a.d; ++f) {
+        double v = scaled(a, i, f);
+        if (u64) u64[i * a.d + f] = v;
+        if (s64) s64[f * npad + i] = v;
+        if (s32) {
+            double c = 0.5 * (key_dbl(mm[2 * f]) + key_dbl(mm[2 * f + 1]));
+            double p = v - c;
+            float pf = (float)p;
+            s32[f * npad + i] = pf;
+            nrm += (double)pf * (double)pf;
+            pp += p * p;
+        }
+    }
+    if (s32) {
+        s32[dp * npad + i] = -(float)nrm;
+        if (p2) p2[i] = pp;
+        if (smax) {
+            float r = (float)sqrt(fmax(nrm, pp)) * 1.0001f;
+            atomicMax(smax, __float_as_uint(r));
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Streaming scan (exact FP64 or FP32 screen)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct ScanArgs {
+    const T* ref;        // SoA: rows 0..d-1 coordinates, (screen) row d = -|r|^2
+    int64_t ref_pad;
+    int64_t n_ref;
+    const T* qry;        // SoA coordinates, qry_pad columns
+    int64_t qry_pad;
+    int64_t n_query;
+    int has_cut;
+    int64_t cut;
+    int reverse;         // process the longest rows first
+    int k_out;           // exact: neighbours written per row (<= K)
+    int32_t* out_idx;    // exact: n_query x k_out ; screen: n_query x K candidates
+    float* out_thr;      // screen: K-th candidate score per row
+};
+
+__device__ __forceinline__ int64_t wave_min64(int64_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        int64_t o = __shfl_xor(v, off, 64);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+__device__ __forceinline__ int64_t wave_max64(int64_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        int64_t o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+template <typename T, int K>
+__device__ __forceinline__ void topk_insert(T (&keys)[K], int (&ids)[K], T xk, int xi) {
+    // Compare-exchange ladder over a sorted register list.  New entries come in
+    // increasing ref index, so a strict '<' keeps equal keys in index order.
+#pragma unroll
+    for (int t = 0; t < K; ++t) {
+        bool lt = xk < keys[t];
+        T nk = lt ? xk : keys[t];
+        T ok = lt ? keys[t] : xk;
+        int ni = lt ? xi : ids[t];
+        int oi = lt ? ids[t] : xi;
+        keys[t] = nk;
+        ids[t] = ni;
+        xk = ok;
+        xi = oi;
+    }
+}
+
+template <typename T, int D>
+__device__ __forceinline__ T exact_score(const T* ref, int64_t pad, int64_t r, const T (&qr)[D]) {
+    T t0 = ref[r] - qr[0];
+    T acc = t0 * t0;
+#pragma unroll
+    for (int f = 1; f < D; ++f) {
+        T t = ref[f * pad + r] - qr[f];
+        acc = fma(t, t, acc);
+    }
+    return acc;
+}
+
+template <typename T, int D, int K, int B, int U, bool SCREEN>
+__global__ __launch_bounds__(64) void scan_kernel(ScanArgs<T> a) {
+    __shared__ T sbk[B][64];
+    __shared__ int sbi[B][64];
+    const int lane = threadIdx.x;
+    const int64_t blk = a.reverse ? (int64_t)(gridDim.x - 1 - blockIdx.x) : (int64_t)blockIdx.x;
+    const int64_t q = blk * 64 + lane;
+    const bool qv = q < a.n_query;
+    int64_t lim = a.n_ref;
+    if (a.has_cut) {
+        lim = q + a.cut;
+        lim = lim < 0 ? 0 : (lim > a.n_ref ? a.n_ref : lim);
+    }
+    if (!qv) lim = 0;
+    const int64_t lim_lo = wave_min64(qv ? lim : a.n_ref);
+    const int64_t lim_hi = wave_max64(lim);
+
+    T qr[D];
+    const int64_t qc = qv ? q : 0;
+#pragma unroll
+    for (int f = 0; f < D; ++f) {
+        T v = a.qry[f * a.qry_pad + qc];
+        qr[f] = SCREEN ? (T)(-2) * v : v;
+    }
+    T keys[K];
+    int ids[K];
+#pragma unroll
+    for (int t = 0; t < K; ++t) {
+        keys[t] = dev_inf<T>();
+        ids[t] = -1;
+    }
+    T thr = qv ? dev_inf<T>() : -dev_inf<T>();
+    int cnt = 0;
+
+    auto merge = [&]() {
+#pragma unroll 1
+        for (int b = 0; b < B; ++b) {
+            if (b < cnt) {
+                T xk = sbk[b][lane];
+                int xi = sbi[b][lane];
+                if (xk < keys[K - 1]) topk_insert<T, K>(keys, ids, xk, xi);
+            }
+        }
+        cnt = 0;
+        thr = qv ? keys[K - 1] : -dev_inf<T>();
+    };
+
+    for (int64_t r0 = 0; r0 < lim_hi; r0 += U) {
+        const bool full = (r0 + U) <= lim_lo;
+        unsigned hits = 0u;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t r = r0 + u;
+            bool hit;
+            if constexpr (SCREEN) {
+                T acc = -thr;
+#pragma unroll
+                for (int f = 0; f < D; ++f) acc = fma(qr[f], a.ref[f * a.ref_pad + r], acc);
+                hit = acc < a.ref[D * a.ref_pad + r];
+            } else {
+                hit = exact_score<T, D>(a.ref, a.ref_pad, r, qr) < thr;
+            }
+            if (!full) hit = hit && (r < lim);
+            hits |= hit ? (1u << u) : 0u;
+        }
+        if (__any(hits != 0u)) {
+#pragma unroll 1
+            for (int u = 0; u < U; ++u) {
+                if ((hits >> u) & 1u) {
+                    const int64_t r = r0 + u;
+                    T s;
+                    if constexpr (SCREEN) {
+                        s = -a.ref[D * a.ref_pad + r];
+#pragma unroll
+                        for (int f = 0; f < D; ++f) s = fma(qr[f], a.ref[f * a.ref_pad + r], s);
+                    } else {
+                        s = exact_score<T, D>(a.ref, a.ref_pad, r, qr);
+                    }
+                    sbk[cnt][lane] = s;
+                    sbi[cnt][lane] = (int)r;
+                    ++cnt;
+                }
+                if (__any(cnt == B)) merge();
+            }
+        }
+    }
+    if (__any(cnt > 0)) merge();
+    if (!qv) return;
+    if constexpr (SCREEN) {
+#pragma unroll
+        for (int t = 0; t < K; ++t) a.out_idx[q * K + t] = ids[t];
+        a.out_thr[q] = (float)keys[K - 1];
+    } else {
+#pragma unroll
+        for (int t = 0; t < K; ++t)
+            if (t < a.k_out) a.out_idx[q * a.k_out + t] = ids[t];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Re-rank of screened candidates in FP64 + certificate
+// ---------------------------------------------------------------------------
+struct RerankArgs {
+    const double* ru64;   // ref AoS scaled
+    const double* qu64;   // query AoS scaled
+    const double* qp2;    // |u_q - c|^2
+    const unsigned int* smax_bits;
+    int d;
+    int64_t n_query;
+    int k;                // neighbours to output
+    const int32_t* cand;  // n_query x K
+    const float* thr;
+    int32_t* out_idx;     // n_query x k
+    int64_t* flag_rows;   // flagged rows list
+    unsigned long long* flag_count;
+    double eps_factor;    // (4 d + 16) * 2^-24
+};
+
+__device__ __forceinline__ double exact_key(const double* ur, const double* uq, int d) {
+    double t0 = ur[0] - uq[0];
+    double key = t0 * t0;
+    for (int f = 1; f < d; ++f) {
+        double t = ur[f] - uq[f];
+        key = fma(t, t, key);
+    }
+    return key;
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void rerank_kernel(RerankArgs a) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= a.n_query) return;
+    double key[K];
+    int ids[K];
+    int count = 0;
+    const double* uq = a.qu64 + q * a.d;
+#pragma unroll
+    for (int t = 0; t < K; ++t) {
+        int id = a.cand[q * K + t];
+        ids[t] = id;
+        if (id >= 0) {
+            key[t] = exact_key(a.ru64 + (int64_t)id * a.d, uq, a.d);
+            ++count;
+        } else {
+            key[t] = __builtin_inf();
+        }
+    }
+    double dm = -__builtin_inf();
+#pragma unroll
+    for (int t = 0; t < K; ++t) {
+        if (ids[t] < 0) continue;
+        int rank = 0;
+#pragma unroll
+        for (int s = 0; s < K; ++s) {
+            bool before = ids[s] >= 0 &&
+                          (key[s] < key[t] || (key[s] == key[t] && ids[s] < ids[t]));
+            rank += before ? 1 : 0;
+        }
+        if (rank < a.k) a.out_idx[q * a.k + rank] = ids[t];
+        if (rank == a.k - 1) dm = key[t];
+    }
+    for (int j = count; j < a.k; ++j) a.out_idx[q * a.k + j] = -1;
+    if (count < K) return;  // every ref of the row was a candidate: exact
+    // Certificate: every non-candidate r had FP32 score >= thr, so its true key is
+    // >= thr + |p|^2 - E.  Strictly above the m-th candidate key => exact top-m.
+    const double s = (double)__uint_as_float(*a.smax_bits);
+    const double p = sqrt(a.qp2[q]);
+    const double mag = (p + s) * (p + s);
+    const double bound = (double)a.thr[q] + a.qp2[q] - a.eps_factor * mag - 1e-12 * mag;
+    if (!(dm < bound)) {
+        unsigned long long slot = atomicAdd(a.flag_count, 1ULL);
+        a.flag_rows[slot] = q;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-row exact fallback (one thread per row, sorted list in global memory).
+// Used for rows that failed the certificate, and for k > 64 or d > 8.
+// ---------------------------------------------------------------------------
+struct SimpleArgs {
+    const double* ru64;
+    const double* qu64;
+    int d;
+    int64_t n_ref;
+    int64_t n_query;
+    int has_cut;
+    int64_t cut;
+    int k;
+    const int64_t* rows;                   // NULL: row = slot
+    const unsigned long long* row_count;   // NULL: n_query rows
+    double* lkey;                          // k x nthreads
+    int32_t* lidx;                         // k x nthreads
+    int32_t* out_idx;                      // n_query x k
+};
+
+__global__ __launch_bounds__(256) void simple_knn_kernel(SimpleArgs a) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nth = (int64_t)gridDim.x * blockDim.x;
+    const int64_t nrows = a.row_count ? (int64_t)*a.row_count : a.n_query;
+    for (int64_t slot = tid; slot < nrows; slot += nth) {
+        const int64_t q = a.rows ? a.rows[slot] : slot;
+        int64_t lim = a.n_ref;
+        if (a.has_cut) {
+            lim = q + a.cut;
+            lim = lim < 0 ? 0 : (lim > a.n_ref ? a.n_ref : lim);
+        }
+        for (int j = 0; j < a.k; ++j) {
+            a.lkey[j * nth + tid] = __builtin_inf();
+            a.lidx[j * nth + tid] = -1;
+        }
+        double kth = __builtin_inf();
+        const double* uq = a.qu64 + q * a.d;
+        for (int64_t r = 0; r < lim; ++r) {
+            double key = exact_key(a.ru64 + r * a.d, uq, a.d);
+            if (key < kth) {
+                int ii = a.k - 1;
+                while (ii > 0) {
+                    double prev = a.lkey[(ii - 1) * nth + tid];
+                    if (prev <= key) break;
+                    a.lkey[ii * nth + tid] = prev;
+                    a.lidx[ii * nth + tid] = a.lidx[(ii - 1) * nth + tid];
+                    --ii;
+                }
+                a.lkey[ii * nth + tid] = key;
+                a.lidx[ii * nth + tid] = (int32_t)r;
+                kth = a.lkey[(a.k - 1) * nth + tid];
+            }
+        }
+        for (int j = 0; j < a.k; ++j) a.out_idx[q * a.k + j] = a.lidx[j * nth + tid];
+    }
+}
+
+__global__ void add_count_kernel(const unsigned long long* src, int64_t* dst) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *dst += (int64_t)*src;
+}
+
+__global__ void zero_u64_kernel(unsigned long long* p, int n) {
+    int i = threadIdx.x;
+    if (i < n) p[i] = 0ULL;
+}
+
+__global__ void fill_i32_kernel(int32_t* p, int64_t n, int32_t v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Host dispatch
+// ---------------------------------------------------------------------------
+inline int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+// Screen candidate count for k neighbours (multiple of 16, <= 80).
+inline int screen_K(int k) {
+    int K = (int)round_up(k + 16, 16);
+    return K;
+}
+
+struct KnnLayout {
+    WsHeader* hdr;
+    long long* mm;
+    unsigned int* smax;
+    unsigned long long* flag_count;
+    double* ru64;
+    double* rs64;
+    float* rs32;
+    double* rp2;
+    double* qu64;
+    double* qs64;
+    float* qs32;
+    double* qp2;
+    int32_t* cand;
+    float* thr;
+    int64_t* flag_rows;
+    double* lkey;
+    int32_t* lidx;
+};
+
+bool use_fast_scan(int d, int k) { return d >= 1 && d <= 8 && k >= 1 && k <= 64; }
+
+// Feature count the scan kernels are compiled for; extra features are zero in
+// both refs and queries, which leaves every key unchanged (t = 0 adds fma(0,0,acc)).
+inline int scan_dim(int d) { return d <= 3 ? 3 : (d == 4 ? 4 : 8); }
+
+// Exact-scan list length for k neighbours.
+inline int exact_K(int k) { return k <= 16 ? 16 : (k <= 32 ? 32 : 64); }
+
+KnnLayout carve_knn(void* ws, size_t bytes, int64_t n_ref, int64_t n_query, int d, int k,
+                    int precision, bool same, Carver& c) {
+    KnnLayout L{};
+    const int64_t rpad = round_up(std::max<int64_t>(n_ref, 1), 64) + 64;
+    const int64_t qpad = round_up(std::max<int64_t>(n_query, 1), 64) + 64;
+    const bool fast = use_fast_scan(d, k);
+    const bool screen = fast && precision == NNGP_PRECISION_FP32_REFINED;
+    L.hdr = c.take<WsHeader>(1);
+    L.mm = c.take<long long>(2 * NNGP_MAX_DIM);
+    L.smax = c.take<unsigned int>(4);
+    L.flag_count = c.take<unsigned long long>(4);
+    const int dp = scan_dim(d);
+    L.ru64 = c.take<double>((size_t)n_ref * d + 1);
+    if (fast && !screen) L.rs64 = c.take<double>((size_t)rpad * dp);
+    if (screen) {
+        L.rs32 = c.take<float>((size_t)rpad * (dp + 1));
+        L.rp2 = c.take<double>((size_t)n_ref + 1);
+    }
+    if (!same) {
+        L.qu64 = c.take<double>((size_t)n_query * d + 1);
+        if (fast && !screen) L.qs64 = c.take<double>((size_t)qpad * dp);
+        if (screen) {
+            L.qs32 = c.take<float>((size_t)qpad * (dp + 1));
+            L.qp2 = c.take<double>((size_t)n_query + 1);
+        }
+    }
+    if (screen) {
+        const int K = screen_K(k);
+        L.cand = c.take<int32_t>((size_t)n_query * K + 1);
+        L.thr = c.take<float>((size_t)n_query + 1);
+        L.flag_rows = c.take<int64_t>((size_t)n_query + 1);
+    }
+    L.lkey = c.take<double>((size_t)k * kFallbackThreads);
+    L.lidx = c.take<int32_t>((size_t)k * kFallbackThreads);
+    (void)ws;
+    (void)bytes;
+    return L;
+}
+
+template <typename T, int D, int K, bool SCREEN>
+void launch_scan_u(const ScanArgs<T>& a, hipStream_t st) {
+    constexpr int U = (D <= 4) ? 8 : 4;
+    constexpr int B = 16;
+    const int64_t nblk = (a.n_query + 63) / 64;
+    hipLaunchKernelGGL((scan_kernel<T, D, K, B, U, SCREEN>), dim3((unsigned)nblk), dim3(64), 0,
+                       st, a);
+}
+
+template <typename T, int D, bool SCREEN>
+int launch_scan_d(const ScanArgs<T>& a, int K, hipStream_t st) {
+    if constexpr (SCREEN) {
+        switch (K) {
+            case 32: launch_scan_u<T, D, 32, SCREEN>(a, st); return 0;
+            case 48: launch_scan_u<T, D, 48, SCREEN>(a, st); return 0;
+            case 64: launch_scan_u<T, D, 64, SCREEN>(a, st); return 0;
+            case 80: launch_scan_u<T, D, 80, SCREEN>(a, st); return 0;
+            default: return -1;
+        }
+    } else {
+        switch (K) {
+            case 16: launch_scan_u<T, D, 16, SCREEN>(a, st); return 0;
+            case 32: launch_scan_u<T, D, 32, SCREEN>(a, st); return 0;
+            case 64: launch_scan_u<T, D, 64, SCREEN>(a, st); return 0;
+            default: return -1;
+        }
+    }
+}
+
+template <typename T, bool SCREEN>
+int launch_scan(const ScanArgs<T>& a, int d, int K, hipStream_t st) {
+    switch (scan_dim(d)) {
+        case 3: return launch_scan_d<T, 3, SCREEN>(a, K, st);
+        case 4: return launch_scan_d<T, 4, SCREEN>(a, K, st);
+        case 8: return launch_scan_d<T, 8, SCREEN>(a, K, st);
+        default: return -1;
+    }
+}
+
+int launch_rerank(const RerankArgs& a, int K, hipStream_t st) {
+    const unsigned grid = (unsigned)((a.n_query + 255) / 256);
+    switch (K) {
+        case 32: hipLaunchKernelGGL(rerank_kernel<32>, dim3(grid), dim3(256), 0, st, a); return 0;
+        case 48: hipLaunchKernelGGL(rerank_kernel<48>, dim3(grid), dim3(256), 0, st, a); return 0;
+        case 64: hipLaunchKernelGGL(rerank_kernel<64>, dim3(grid), dim3(256), 0, st, a); return 0;
+        case 80: hipLaunchKernelGGL(rerank_kernel<80>, dim3(grid), dim3(256), 0, st, a); return 0;
+        default: return -1;
+    }
+}
+
+}  // namespace
+
+size_t knn_workspace_bytes(int64_t n_ref, int64_t n_query, int d, int k, int precision,
+                           bool same) {
+    Carver c(nullptr, 0);
+    carve_knn(nullptr, 0, n_ref, n_query, d, k, precision, same, c);
+    return c.used + 256;
+}
+
+int knn_run(const double* ref, int64_t n_ref, const double* query, int64_t n_query, int d,
+            const double* scale, int n_scale, int k, int has_cut, int64_t ref_cut, int precision,
+            int32_t* out_idx, void* ws, size_t ws_bytes, hipStream_t st, bool reset_stats) {
+    if (d < 1 || d > NNGP_MAX_DIM || k < 1 || k > NNGP_MAX_NN || n_ref < 0 || n_query < 0)
+        return NNGP_ERR_INVALID_ARGUMENT;
+    if (n_scale != 0 && n_scale != 1 && n_scale != d) return NNGP_ERR_INVALID_ARGUMENT;
+    if (n_ref >= (int64_t)INT32_MAX) return NNGP_ERR_UNSUPPORTED;
+    if (precision != NNGP_PRECISION_FP64 && precision != NNGP_PRECISION_FP32_REFINED)
+        return NNGP_ERR_INVALID_ARGUMENT;
+    const bool same = (ref == query && n_ref == n_query);
+    Carver c(ws, ws_bytes);
+    KnnLayout L = carve_knn(ws, ws_bytes, n_ref, n_query, d, k, precision, same, c);
+    if (!c.ok() || ws == nullptr) return NNGP_ERR_WORKSPACE_TOO_SMALL;
+    if (reset_stats) hipLaunchKernelGGL(zero_header_kernel, dim3(1), dim3(64), 0, st, L.hdr);
+    if (n_query == 0) return NNGP_OK;
+    if (n_ref == 0) {
+        const int64_t tot = n_query * (int64_t)k;
+        hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                           out_idx, tot, -1);
+        NNGP_HIP_CHECK(hipGetLastError());
+        return NNGP_OK;
+    }
+    const bool fast = use_fast_scan(d, k);
+    const bool screen = fast && precision == NNGP_PRECISION_FP32_REFINED;
+    const int64_t rpad = round_up(n_ref, 64) + 64;
+    const int64_t qpad = round_up(n_query, 64) + 64;
+
+    PrepArgs pr{};
+    pr.d = d;
+    pr.n_scale = n_scale;
+    for (int f = 0; f < n_scale; ++f) pr.scale[f] = scale[f];
+
+    hipLaunchKernelGGL(init_minmax_kernel, dim3(1), dim3(64), 0, st, L.mm, d, L.smax);
+    if (screen) {
+        pr.x = ref;
+        pr.n = n_ref;
+        hipLaunchKernelGGL(minmax_kernel, dim3(256), dim3(256), 0, st, pr, L.mm);
+        if (!same) {
+            pr.x = query;
+            pr.n = n_query;
+            hipLaunchKernelGGL(minmax_kernel, dim3(256), dim3(256), 0, st, pr, L.mm);
+        }
+    }
+    pr.x = ref;
+    pr.n = n_ref;
+    const int dp = scan_dim(d);
+    hipLaunchKernelGGL(prep_kernel, dim3((unsigned)((rpad + 255) / 256)), dim3(256), 0, st, pr,
+                       L.mm, rpad, dp, L.ru64, L.rs64, L.rs32, L.rp2, L.smax);
+    const double* qu64 = L.ru64;
+    const double* qs64 = L.rs64;
+    const float* qs32 = L.rs32;
+    const double* qp2 = L.rp2;
+    if (!same) {
+        pr.x = query;
+        pr.n = n_query;
+        hipLaunchKernelGGL(prep_kernel, dim3((unsigned)((qpad + 255) / 256)), dim3(256), 0, st, pr,
+                           L.mm, qpad, dp, L.qu64, L.qs64, L.qs32, L.qp2, L.smax);
+        qu64 = L.qu64;
+        qs64 = L.qs64;
+        qs32 = L.qs32;
+        qp2 = L.qp2;
+    }
+    NNGP_HIP_CHECK(hipGetLastError());
+
+    if (!fast) {
+        SimpleArgs sa{};
+        sa.ru64 = L.ru64;
+        sa.qu64 = qu64;
+        sa.d = d;
+        sa.n_ref = n_ref;
+        sa.n_query = n_query;
+        sa.has_cut = has_cut;
+        sa.cut = ref_cut;
+        sa.k = k;
+        sa.lkey = L.lkey;
+        sa.lidx = L.lidx;
+        sa.out_idx = out_idx;
+        hipLaunchKernelGGL(simple_knn_kernel, dim3(kFallbackThreads / 256), dim3(256), 0, st, sa);
+        NNGP_HIP_CHECK(hipGetLastError());
+        return NNGP_OK;
+    }
+
+    if (!screen) {
+        ScanArgs<double> a{};
+        a.ref = L.rs64;
+        a.ref_pad = rpad;
+        a.n_ref = n_ref;
+        a.qry = qs64;
+        a.qry_pad = same ? rpad : qpad;
+        a.n_query = n_query;
+        a.has_cut = has_cut;
+        a.cut = ref_cut;
+        a.reverse = has_cut;
+        a.k_out = k;
+        a.out_idx = out_idx;
+        if (launch_scan<double, false>(a, d, exact_K(k), st) != 0) return NNGP_ERR_UNSUPPORTED;
+        NNGP_HIP_CHECK(hipGetLastError());
+        return NNGP_OK;
+    }
+
+    const int K = screen_K(k);
+    ScanArgs<float> a{};
+    a.ref = L.rs32;
+    a.ref_pad = rpad;
+    a.n_ref = n_ref;
+    a.qry = qs32;
+    a.qry_pad = same ? rpad : qpad;
+    a.n_query = n_query;
+    a.has_cut = has_cut;
+    a.cut = ref_cut;
+    a.reverse = has_cut;
+    a.out_idx = L.cand;
+    a.out_thr = L.thr;
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, L.flag_count, 1);
+    if (launch_scan<float, true>(a, d, K, st) != 0) return NNGP_ERR_UNSUPPORTED;
+
+    RerankArgs rr{};
+    rr.ru64 = L.ru64;
+    rr.qu64 = qu64;
+    rr.qp2 = qp2;
+    rr.smax_bits = L.smax;
+    rr.d = d;
+    rr.n_query = n_query;
+    rr.k = k;
+    rr.cand = L.cand;
+    rr.thr = L.thr;
+    rr.out_idx = out_idx;
+    rr.flag_rows = L.flag_rows;
+    rr.flag_count = L.flag_count;
+    rr.eps_factor = (4.0 * d + 16.0) * std::ldexp(1.0, -24);
+    if (launch_rerank(rr, K, st) != 0) return NNGP_ERR_UNSUPPORTED;
+
+    SimpleArgs sa{};
+    sa.ru64 = L.ru64;
+    sa.qu64 = qu64;
+    sa.d = d;
+    sa.n_ref = n_ref;
+    sa.n_query = n_query;
+    sa.has_cut = has_cut;
+    sa.cut = ref_cut;
+    sa.k = k;
+    sa.rows = L.flag_rows;
+    sa.row_count = L.flag_count;
+    sa.lkey = L.lkey;
+    sa.lidx = L.lidx;
+    sa.out_idx = out_idx;
+    hipLaunchKernelGGL(simple_knn_kernel, dim3(kFallbackThreads / 256), dim3(256), 0, st, sa);
+    hipLaunchKernelGGL(add_count_kernel, dim3(1), dim3(1), 0, st, L.flag_count,
+                       &L.hdr->knn_rows_recomputed);
+    NNGP_HIP_CHECK(hipGetLastError());
+    return NNGP_OK;
+}
+
+}  // namespace nngp

@@ -1,0 +1,153 @@
+"""Reduction of a kernel tree to the accelerated form  White + C + s * RBF.
+
+Restates the reference's RbfAccelerator.get_kernel_params and
+get_kernel_theta_mapping (cuda_kernel_accelerators.py:520-649) and the
+CudaKernelAccelerator cache (:18-164), with one deliberate difference: Matern is a
+subclass of RBF in scikit-learn, and the reference's subclass test lets it through
+as if it were an RBF; here it is rejected (NotImplementedError) like any other
+non-accelerated kernel.
+"""
+import copy
+import numbers
+
+import numpy as np
+from sklearn.gaussian_process import kernels
+
+
+def get_kernel_params(kernel):
+    """Dict of {noise_level, constant_value, rbf_level, length_scale} describing
+    the kernel as  noise*[i==j] + constant + rbf_level*RBF(length_scale).
+
+    Raises NotImplementedError for kernels outside that family
+    (cuda_kernel_accelerators.py:534-590).
+    """
+    if isinstance(kernel, kernels.Matern):
+        raise NotImplementedError("Matern kernels are not accelerated")
+    if isinstance(kernel, kernels.WhiteKernel):
+        return {"noise_level": kernel.noise_level}
+    if isinstance(kernel, kernels.ConstantKernel):
+        return {"constant_value": kernel.constant_value}
+    if isinstance(kernel, kernels.RBF):
+        ls = kernel.length_scale
+        return {"length_scale": ls if isinstance(ls, numbers.Number)
+                else np.array(ls).reshape((1, -1))}
+    if isinstance(kernel, kernels.Sum):
+        p1 = get_kernel_params(kernel.k1)
+        p2 = get_kernel_params(kernel.k2)
+        rbf1 = len(set(p1).difference({"noise_level", "constant_value"})) > 0
+        rbf2 = len(set(p2).difference({"noise_level", "constant_value"})) > 0
+        if rbf1 and rbf2:
+            raise NotImplementedError("Sum of two RBF-type kernels is not supported: %s + %s"
+                                      % (p1, p2))
+        if rbf1:
+            p1, p2 = p2, p1
+        if "noise_level" in p1 or "noise_level" in p2:
+            p2["noise_level"] = p1.get("noise_level", 0) + p2.get("noise_level", 0)
+        if "constant_value" in p1 or "constant_value" in p2:
+            p2["constant_value"] = p1.get("constant_value", 0) + p2.get("constant_value", 0)
+        return p2
+    if isinstance(kernel, kernels.Product):
+        p1 = get_kernel_params(kernel.k1)
+        p2 = get_kernel_params(kernel.k2)
+        c1 = len(set(p1).difference({"constant_value"})) == 0
+        c2 = len(set(p2).difference({"constant_value"})) == 0
+        if not c1 and not c2:
+            raise NotImplementedError("Product is supported only with a ConstantKernel: %s * %s"
+                                      % (p1, p2))
+        if c2:
+            p1, p2 = p2, p1
+        if "noise_level" in p2:
+            p2["noise_level"] = p1["constant_value"] * p2["noise_level"]
+        if "constant_value" in p2:
+            p2["constant_value"] = p1["constant_value"] * p2["constant_value"]
+        if "rbf_level" in p2 or "length_scale" in p2:
+            p2["rbf_level"] = p1["constant_value"] * p2.get("rbf_level", 1)
+        return p2
+    raise NotImplementedError("Unsupported kernel type: " + str(type(kernel)))
+
+
+def get_kernel_theta_mapping(kernel):
+    """Index in kernel.theta of each parameter of get_kernel_params.
+
+    Clones the kernel with theta = arange(n) and matches every parameter against
+    exp(i) within a relative 1e-15 (cuda_kernel_accelerators.py:610-649).  Every
+    theta must be matched, ARD indices must be contiguous.
+    """
+    n_theta = len(kernel.theta)
+    epsi = 1e-15
+    probe = kernel.clone_with_theta(np.arange(n_theta, dtype=np.float64))
+    expected = np.exp(probe.theta)
+    params = get_kernel_params(probe)
+
+    def find(name):
+        val = params[name]
+        if name != "length_scale" or isinstance(val, numbers.Number):
+            found = (val < expected * (1 + epsi)) & (val > expected * (1 - epsi))
+            nf = int(np.sum(found))
+            if nf > 1:
+                raise NotImplementedError("Cannot find theta mapping for parameter " + name)
+            if nf == 0:
+                return None
+            return int(np.where(found)[0][0])
+        vals = np.asarray(val).reshape(-1)
+        found = np.zeros(len(expected), dtype=bool)
+        for v in vals:
+            hit = (v < expected * (1 + epsi)) & (v > expected * (1 - epsi))
+            if np.sum(hit) == 1:
+                found[np.where(hit)[0][0]] = True
+        nf = int(np.sum(found))
+        if nf > vals.size:
+            raise NotImplementedError("Cannot find theta mapping for parameter " + name)
+        if nf == 0:
+            return None
+        ind = np.where(found)[0]
+        if len(ind) > 1 and set(np.unique(np.diff(ind))) != {1}:
+            raise NotImplementedError("Unsupported case of non-adjacent length-scale thetas")
+        return ind
+
+    mapping = {p: find(p) for p in params}
+    mapping = {k: v for k, v in mapping.items() if v is not None}
+    n_found = sum(len(v) if isinstance(v, np.ndarray) else 1 for v in mapping.values())
+    if n_found != n_theta:
+        raise NotImplementedError("Cannot find the mapping for all kernel thetas")
+    return mapping
+
+
+def can_accelerate_kernel(kernel):
+    try:
+        get_kernel_params(kernel)
+        get_kernel_theta_mapping(kernel)
+    except NotImplementedError:
+        return False
+    return True
+
+
+class KernelDecomposition:
+    """Cache of the decomposition of the last kernel seen (identity + theta),
+    like CudaKernelAccelerator._update_kernel_if_needed (cuda_kernel_accelerators.py:146-164)."""
+
+    def __init__(self):
+        self._kernel = None
+        self._theta = None
+        self.params = None
+        self.mapping = None
+        self.error = None
+
+    def update(self, kernel):
+        theta = np.asarray(kernel.theta)
+        if self._kernel is kernel and self._theta is not None and self._theta.shape == theta.shape \
+                and np.all(self._theta == theta):
+            return self
+        self._kernel = kernel
+        self._theta = copy.deepcopy(theta)
+        self.params, self.mapping, self.error = None, None, None
+        try:
+            self.params = get_kernel_params(kernel)
+            self.mapping = get_kernel_theta_mapping(kernel)
+        except NotImplementedError as e:
+            self.params, self.mapping, self.error = None, None, e
+        return self
+
+    @property
+    def accelerated(self):
+        return self.params is not None

@@ -1,0 +1,180 @@
+"""HIP fused Vecchia kernels vs the FP64 oracle and scikit-learn's exact GP.
+
+Tolerances: FP64 form 1e-10 relative per point (1e-9 on gradients); FP32 +
+refinement form |dl|/n <= 1e-6 and mean/std relative error <= 1e-5 (the
+targets stated for that form).
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+from sklearn import gaussian_process as skgp
+from sklearn.gaussian_process import kernels as skk
+
+from nngpr_amd import _native as nat
+from nngpr_amd import datasets, hip_ops
+from nngpr_amd import batched_kernels as bk
+from nngpr_amd.kernel_params import get_kernel_params, get_kernel_theta_mapping
+from oracle import vecchia_oracle as vo
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "small_uniform.npz")
+SU_MAP = {"noise_level": 0, "rbf_level": 1, "length_scale": 2}
+
+
+def t(a, dtype=torch.float64):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(DEV, dtype)
+
+
+def kstruct(w, C, s, ls, theta_map, n_theta):
+    params = {"noise_level": w, "constant_value": C}
+    if ls is not None:
+        params["length_scale"] = ls
+        params["rbf_level"] = s
+    return nat.kernel_struct(params, theta_map, n_theta)
+
+
+def tol(prec):
+    return 1e-10 if prec == 0 else 1e-5
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_small_uniform_loglik_golden(prec):
+    g = np.load(GOLD)
+    ds = datasets.small_uniform()
+    x, y = ds["x_train"], ds["y_train"]
+    ks = kstruct(0.01, 0.0, 1.0, 0.3, SU_MAP, 3)
+    ll, grad, pts = hip_ops.vecchia_loglik(t(x), t(y), t(g["nn_train"], torch.int32), ks, 1e-10,
+                                           precision=prec, eval_gradient=True, point_loglik=True)
+    n = x.shape[0]
+    assert abs(ll.item() - float(g["loglik"])) / n <= (1e-12 if prec == 0 else 1e-6)
+    np.testing.assert_allclose(pts.cpu().numpy(), g["loglik_points"], rtol=tol(prec), atol=tol(prec))
+    np.testing.assert_allclose(grad.cpu().numpy(), g["grad"], rtol=1e-9 if prec == 0 else 1e-5)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_small_uniform_predict_golden(prec):
+    g = np.load(GOLD)
+    ds = datasets.small_uniform()
+    ks = kstruct(0.01, 0.0, 1.0, 0.3, {}, 0)
+    mean, std = hip_ops.predict(t(ds["x_train"]), t(ds["y_train"]), t(ds["x_test"]),
+                                t(g["nn_test"], torch.int32), ks, 1e-10, precision=prec)
+    np.testing.assert_allclose(mean.cpu().numpy()[:, 0], g["mean"], rtol=tol(prec), atol=tol(prec))
+    np.testing.assert_allclose(std.cpu().numpy(), g["std"], rtol=tol(prec))
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("m", [1, 5, 16, 32, 64, 90])
+def test_loglik_grad_vs_oracle(prec, m):
+    rng = np.random.default_rng(m)
+    x = rng.random((700, 3))
+    y = np.column_stack([np.sin(4 * x[:, 0]), x[:, 1] ** 2]) + rng.normal(0, 0.1, (700, 2))
+    tm = {"noise_level": 0, "constant_value": 1, "rbf_level": 2, "length_scale": 3}
+    w, C, s, ls = 0.05, 0.4, 1.2, 0.25
+    nn = vo.vecchia_knn(x, ls, m)
+    ll_o, g_o, pts_o = vo.vecchia_loglik(x, y, nn, w, C, s, ls, 1e-10, tm, 4, eval_gradient=True)
+    ks = kstruct(w, C, s, ls, tm, 4)
+    ll, g, pts = hip_ops.vecchia_loglik(t(x), t(y), t(nn, torch.int32), ks, 1e-10, precision=prec,
+                                        eval_gradient=True, point_loglik=True)
+    assert abs(ll.item() - ll_o) / x.shape[0] <= (1e-12 if prec == 0 else 1e-6)
+    np.testing.assert_allclose(pts.cpu().numpy(), pts_o, rtol=tol(prec), atol=tol(prec))
+    np.testing.assert_allclose(g.cpu().numpy(), g_o, rtol=1e-9 if prec == 0 else 1e-5,
+                               atol=1e-9 if prec == 0 else 1e-5)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_ard_grad_vs_oracle(prec):
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=(500, 6))
+    y = np.sin(x[:, 0]) + 0.3 * x[:, 3] + rng.normal(0, 0.05, 500)
+    ls = rng.uniform(0.8, 3.0, size=6)
+    tm = {"noise_level": 0, "rbf_level": 1, "length_scale": np.arange(2, 8)}
+    nn = vo.vecchia_knn(x, ls, 12)
+    ll_o, g_o, _ = vo.vecchia_loglik(x, y, nn, 0.02, 0.0, 0.9, ls, 1e-10, tm, 8, eval_gradient=True)
+    ll, g, _ = hip_ops.vecchia_loglik(t(x), t(y), t(nn, torch.int32), kstruct(0.02, 0.0, 0.9, ls, tm, 8),
+                                      1e-10, precision=prec, eval_gradient=True)
+    assert abs(ll.item() - ll_o) / 500 <= (1e-12 if prec == 0 else 1e-6)
+    np.testing.assert_allclose(g.cpu().numpy(), g_o, rtol=1e-9 if prec == 0 else 1e-5, atol=1e-8)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_predict_vs_oracle_multioutput(prec):
+    rng = np.random.default_rng(8)
+    x = rng.random((1500, 2))
+    y = rng.normal(size=(1500, 3))
+    xq = rng.random((400, 2))
+    nn = vo.knn(x, xq, 0.2, 20)
+    mean_o, std_o = vo.predict(x, y, xq, nn, 0.1, 0.5, 1.5, 0.2, 1e-8)
+    mean, std = hip_ops.predict(t(x), t(y), t(xq), t(nn, torch.int32),
+                                kstruct(0.1, 0.5, 1.5, 0.2, {}, 0), 1e-8, precision=prec)
+    np.testing.assert_allclose(mean.cpu().numpy(), mean_o, rtol=tol(prec), atol=tol(prec))
+    np.testing.assert_allclose(std.cpu().numpy(), std_o, rtol=tol(prec))
+
+
+def test_exact_gp_equivalence_m_ge_n_minus_1():
+    # The reference's known-answer test (nngpr_base_tester.py:174-192) on the HIP path.
+    rng = np.random.RandomState(234058)
+    x = rng.rand(150, 4)
+    rng.rand(50, 4)
+    beta = rng.rand(4, 3)
+    beta[:, -1] += 1
+    y = x @ beta + rng.rand(150, 3)
+    y[:, 0] *= 10
+    kern = skk.ConstantKernel(1) * skk.RBF(1) + skk.WhiteKernel(1)
+    theta = np.full(3, 0.5)
+    gpr = skgp.GaussianProcessRegressor(kernel=kern, alpha=1e-10)
+    gpr.X_train_, gpr.y_train_, gpr.kernel_ = x, y, kern
+    ll_ref, g_ref = gpr.log_marginal_likelihood(theta, eval_gradient=True)
+    k2 = kern.clone_with_theta(theta)
+    params, mapping = get_kernel_params(k2), get_kernel_theta_mapping(k2)
+    ks = nat.kernel_struct(params, mapping, 3)
+    nn = hip_ops.knn(t(x), t(x), np.exp(0.5), 149, ref_cut=0)
+    ll, g, _ = hip_ops.vecchia_loglik(t(x), t(y), nn, ks, 1e-10, eval_gradient=True)
+    assert abs(ll.item() / ll_ref - 1) < 1e-10
+    np.testing.assert_allclose(g.cpu().numpy(), g_ref, rtol=1e-9)
+
+
+def test_fill_kernel_vs_oracle():
+    rng = np.random.default_rng(4)
+    x = rng.random((300, 3))
+    nn = rng.integers(-1, 300, size=(50, 9)).astype(np.int32)
+    tm = {"noise_level": 0, "constant_value": 1, "rbf_level": 2, "length_scale": 3}
+    ks = kstruct(0.1, 0.3, 1.4, 0.5, tm, 4)
+    K, G = hip_ops.fill_nn_kernel(t(x), t(nn, torch.int32), ks, eval_gradient=True)
+    valid = nn >= 0
+    Ko, Go = vo.fill(x[np.where(valid, nn, 0)], valid, 0.1, 0.3, 1.4, 0.5, tm, 4, True)
+    np.testing.assert_allclose(K.cpu().numpy(), Ko, rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(G.cpu().numpy(), Go, rtol=1e-12, atol=1e-15)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_not_positive_definite_raises(prec):
+    x = np.repeat(np.random.default_rng(1).random((20, 2)), 3, axis=0)
+    y = np.zeros(60)
+    nn = vo.vecchia_knn(x, 1.0, 4)
+    ks = kstruct(0.0, 0.0, 1.0, 1.0, {}, 0)
+    with pytest.raises(np.linalg.LinAlgError):
+        hip_ops.vecchia_loglik(t(x), t(y), t(nn, torch.int32), ks, 0.0, precision=prec)
+
+
+def test_gradient_central_differences_on_gpu():
+    rng = np.random.default_rng(21)
+    x = rng.random((400, 3))
+    y = np.cos(3 * x[:, 0]) * x[:, 2] + rng.normal(0, 0.1, 400)
+    kern = bk.WhiteKernel(0.05) + bk.ConstantKernel(0.8) * bk.RBF(0.3)
+    nn = vo.vecchia_knn(x, 0.3, 10)
+    nnt = t(nn, torch.int32)
+
+    def f(theta, grad=False):
+        k = kern.clone_with_theta(theta)
+        ks = nat.kernel_struct(get_kernel_params(k), get_kernel_theta_mapping(k), 3)
+        ll, g, _ = hip_ops.vecchia_loglik(t(x), t(y), nnt, ks, 1e-10, eval_gradient=grad)
+        return (ll.item(), g.cpu().numpy()) if grad else ll.item()
+
+    th = kern.theta
+    _, g = f(th, True)
+    h = 1e-5
+    fd = np.array([(f(th + h * e) - f(th - h * e)) / (2 * h) for e in np.eye(3)])
+    np.testing.assert_allclose(g, fd, rtol=1e-6, atol=1e-6)
