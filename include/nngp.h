@@ -156,6 +156,16 @@ typedef struct {
 } nngp_stats;
 int nngp_read_stats(const void* workspace, nngp_stats* out, void* stream);
 
+/*
+ * Kernel timing with HIP events recorded on the launch stream around the main
+ * kernels: 0 = kNN scan, 1 = kNN re-rank, 2 = fused log-likelihood pass,
+ * 3 = fused prediction pass.  Off by default; nngp_timing_read synchronises on
+ * the recorded events and returns the summed milliseconds and launch count.
+ * No reference counterpart (measurement support).
+ */
+int nngp_timing_enable(int on);
+int nngp_timing_read(int kernel, int reset, double* total_ms, int64_t* launches);
+
 #ifdef __cplusplus
 }
 #endif

@@ -67,7 +67,24 @@ SIGNATURES = [
     ("nngp_predict", _I, [_P, _P, _I64, _I, _I, _P, _I64, _P, _I,
                           ctypes.POINTER(NngpKernel), _D, _I, _P, _P, _P, _SZ, _P]),
     ("nngp_read_stats", _I, [_P, ctypes.POINTER(NngpStats), _P]),
+    ("nngp_timing_enable", _I, [_I]),
+    ("nngp_timing_read", _I, [_I, _I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I64)]),
 ]
+
+TIMED_KNN_SCAN, TIMED_KNN_RERANK, TIMED_LOGLIK, TIMED_PREDICT = 0, 1, 2, 3
+
+
+def timing_enable(on=True):
+    lib().nngp_timing_enable(1 if on else 0)
+
+
+def timing_read(kernel, reset=True):
+    """(total milliseconds, launches) recorded for a timed kernel."""
+    ms = ctypes.c_double()
+    n = ctypes.c_int64()
+    check(lib().nngp_timing_read(kernel, 1 if reset else 0, ctypes.byref(ms), ctypes.byref(n)),
+          "nngp_timing_read")
+    return ms.value, n.value
 
 _lib = None
 

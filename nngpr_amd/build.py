@@ -29,7 +29,7 @@ def build(quiet=False, jobs=None):
     """Compile every .hip source to an object, then link the shared library."""
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    flags = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
+    flags = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-fno-slp-vectorize",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-command-line-argument", "-I", INCLUDE]
     procs = []

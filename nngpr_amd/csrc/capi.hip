@@ -4,10 +4,13 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
+#include <vector>
 
 #include "common.h"
 #include "knn.h"
 #include "vecchia.h"
+#include "vecchia_reg.h"
 
 namespace nngp {
 
@@ -16,6 +19,42 @@ static thread_local char g_last_error[256] = "";
 void set_last_hip_error(hipError_t e, const char* what, int line) {
     std::snprintf(g_last_error, sizeof(g_last_error), "%s (%d) at capi line %d: %s",
                   hipGetErrorString(e), (int)e, line, what);
+}
+
+namespace {
+
+struct TimingState {
+    std::mutex mu;
+    bool on = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[TK_COUNT];
+    hipEvent_t pending[TK_COUNT] = {};
+};
+TimingState& timing() {
+    static TimingState t;
+    return t;
+}
+
+}  // namespace
+
+void timing_begin(int id, hipStream_t st) {
+    TimingState& t = timing();
+    std::lock_guard<std::mutex> lk(t.mu);
+    if (!t.on) return;
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    hipEventRecord(e, st);
+    t.pending[id] = e;
+}
+
+void timing_end(int id, hipStream_t st) {
+    TimingState& t = timing();
+    std::lock_guard<std::mutex> lk(t.mu);
+    if (!t.on || !t.pending[id]) return;
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    hipEventRecord(e, st);
+    t.ev[id].push_back({t.pending[id], e});
+    t.pending[id] = nullptr;
 }
 
 namespace {
@@ -56,9 +95,31 @@ struct VecLayout {
     double* wsL;
     double* wsv;
     int64_t P;
+    int64_t* flag_list;
+    unsigned long long* flag_count;
 };
 
-VecLayout carve_vec(Carver& c, int64_t npts, int m, int n_theta, bool need_pt) {
+// Largest dynamic LDS a block may use on the current device.
+size_t max_block_lds() {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 64 * 1024;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+        return 64 * 1024;
+    return (size_t)v;
+}
+
+// Which fused kernel serves (m, d): the register kernel when its LDS fits.
+bool use_register_kernel(int m, int d, int precision) {
+    if (reg_mr(m) == 0) return false;
+    const bool fp32 = precision == NNGP_PRECISION_FP32_REFINED;
+    // The FP32 form re-runs flagged blocks with the FP64 kernel: both must fit.
+    size_t need = reg_lds_bytes(m, d, false);
+    if (fp32) need = need > reg_lds_bytes(m, d, true) ? need : reg_lds_bytes(m, d, true);
+    return need <= max_block_lds();
+}
+
+VecLayout carve_vec(Carver& c, int64_t npts, int m, int d, int n_theta, bool need_pt,
+                    int precision) {
     VecLayout L{};
     L.hdr = c.take<WsHeader>(1);
     const int cols = 1 + n_theta;
@@ -66,10 +127,56 @@ VecLayout carve_vec(Carver& c, int64_t npts, int m, int n_theta, bool need_pt) {
         L.pt = c.take<double>((size_t)npts * cols + 1);
         L.partial = c.take<double>(reduce_partial_doubles(cols));
     }
-    L.P = generic_chunk(npts, m);
-    L.wsL = c.take<double>((size_t)L.P * ((size_t)m * (m + 1) / 2));
-    L.wsv = c.take<double>((size_t)L.P * 5 * (size_t)m);
+    L.flag_count = c.take<unsigned long long>(4);
+    if (use_register_kernel(m, d, precision)) {
+        L.P = 0;
+        if (precision == NNGP_PRECISION_FP32_REFINED) L.flag_list = c.take<int64_t>((size_t)npts + 1);
+    } else {
+        L.P = generic_chunk(npts, m);
+        L.wsL = c.take<double>((size_t)L.P * ((size_t)m * (m + 1) / 2));
+        L.wsv = c.take<double>((size_t)L.P * 5 * (size_t)m);
+    }
     return L;
+}
+
+__global__ void zero_flag_kernel(unsigned long long* c) {
+    if (threadIdx.x == 0) *c = 0ULL;
+}
+
+__global__ void add_refactored_kernel(const unsigned long long* c, WsHeader* h) {
+    if (threadIdx.x == 0) h->points_refactored_fp64 += (int64_t)*c;
+}
+
+// Runs the fused pass over a.npts points with the kernel that fits (m, d).
+int run_fused(const VecArgs& a, const VecLayout& L, int precision, hipStream_t st) {
+    if (a.npts == 0) return NNGP_OK;
+    if (!use_register_kernel(a.m, a.d, precision)) {
+        launch_generic(a, L.wsL, L.wsv, L.P, st);
+        return NNGP_OK;
+    }
+    RegExtra ex{};
+    const int tk = a.ys ? TK_LOGLIK : TK_PREDICT;
+    if (precision == NNGP_PRECISION_FP32_REFINED) {
+        hipLaunchKernelGGL(zero_flag_kernel, dim3(1), dim3(64), 0, st, L.flag_count);
+        ex.flag_list = L.flag_list;
+        ex.flag_count = L.flag_count;
+        timing_begin(tk, st);
+        const int rc = launch_reg(a, ex, true, a.npts, st);
+        timing_end(tk, st);
+        if (rc) return NNGP_ERR_UNSUPPORTED;
+        RegExtra redo{};
+        redo.plist = L.flag_list;
+        redo.pcount = L.flag_count;
+        const int64_t cap = a.npts < 4096 ? a.npts : 4096;
+        if (launch_reg(a, redo, false, cap, st)) return NNGP_ERR_UNSUPPORTED;
+        hipLaunchKernelGGL(add_refactored_kernel, dim3(1), dim3(64), 0, st, L.flag_count, L.hdr);
+    } else {
+        timing_begin(tk, st);
+        const int rc = launch_reg(a, ex, false, a.npts, st);
+        timing_end(tk, st);
+        if (rc) return NNGP_ERR_UNSUPPORTED;
+    }
+    return NNGP_OK;
 }
 
 __global__ void extract_col_kernel(const double* pt, int stride, int64_t n, double* out) {
@@ -100,6 +207,36 @@ const char* nngp_status_string(int status) {
 const char* nngp_last_error_detail(void) { return g_last_error; }
 
 int nngp_abi_version(void) { return NNGP_ABI_VERSION; }
+
+int nngp_timing_enable(int on) {
+    TimingState& t = timing();
+    std::lock_guard<std::mutex> lk(t.mu);
+    t.on = on != 0;
+    return NNGP_OK;
+}
+
+int nngp_timing_read(int kernel, int reset, double* total_ms, int64_t* launches) {
+    if (kernel < 0 || kernel >= TK_COUNT || !total_ms || !launches) return NNGP_ERR_INVALID_ARGUMENT;
+    TimingState& t = timing();
+    std::lock_guard<std::mutex> lk(t.mu);
+    double tot = 0.0;
+    for (auto& pr : t.ev[kernel]) {
+        NNGP_HIP_CHECK(hipEventSynchronize(pr.second));
+        float ms = 0.0f;
+        NNGP_HIP_CHECK(hipEventElapsedTime(&ms, pr.first, pr.second));
+        tot += ms;
+    }
+    *total_ms = tot;
+    *launches = (int64_t)t.ev[kernel].size();
+    if (reset) {
+        for (auto& pr : t.ev[kernel]) {
+            hipEventDestroy(pr.first);
+            hipEventDestroy(pr.second);
+        }
+        t.ev[kernel].clear();
+    }
+    return NNGP_OK;
+}
 
 size_t nngp_knn_workspace_bytes(int64_t n_ref, int64_t n_query, int d, int k, int precision) {
     // Separate query buffers are always budgeted so one size serves both forms.
@@ -132,12 +269,10 @@ int nngp_fill_nn_kernel(const double* x, int64_t n_x, int d, const int32_t* nn,
 
 size_t nngp_vecchia_workspace_bytes(int64_t n_rows, int m, int d, int ny, int n_theta,
                                     int precision, int eval_gradient) {
-    (void)d;
     (void)ny;
-    (void)precision;
     (void)eval_gradient;
     Carver c(nullptr, 0);
-    carve_vec(c, n_rows, m, n_theta, true);
+    carve_vec(c, n_rows, m, d, n_theta, true, precision);
     return c.used + 256;
 }
 
@@ -159,7 +294,7 @@ int nngp_vecchia_loglik(const double* x, const double* y, int64_t n, int d, int 
     if (eval_gradient && kp.n_theta > 0 && !out_grad) return NNGP_ERR_INVALID_ARGUMENT;
     const int64_t npts = row1 - row0;
     Carver c(workspace, workspace_bytes);
-    VecLayout L = carve_vec(c, npts, m, kp.n_theta, true);
+    VecLayout L = carve_vec(c, npts, m, d, kp.n_theta, true, precision);
     if (!c.ok() || !workspace) return NNGP_ERR_WORKSPACE_TOO_SMALL;
     hipLaunchKernelGGL(zero_header_kernel, dim3(1), dim3(64), 0, st, L.hdr);
     VecArgs a{};
@@ -180,7 +315,8 @@ int nngp_vecchia_loglik(const double* x, const double* y, int64_t n, int d, int 
     a.pt = L.pt;
     a.pt_stride = 1 + kp.n_theta;
     a.hdr = L.hdr;
-    if (npts > 0) launch_generic(a, L.wsL, L.wsv, L.P, st);
+    rc = run_fused(a, L, precision, st);
+    if (rc) return rc;
     launch_reduce(L.pt, npts, 1 + kp.n_theta, L.partial, out_loglik,
                   eval_gradient ? out_grad : nullptr, st);
     if (out_point_loglik && npts > 0)
@@ -191,11 +327,9 @@ int nngp_vecchia_loglik(const double* x, const double* y, int64_t n, int d, int 
 }
 
 size_t nngp_predict_workspace_bytes(int64_t n_query, int m, int d, int ny, int precision) {
-    (void)d;
     (void)ny;
-    (void)precision;
     Carver c(nullptr, 0);
-    carve_vec(c, n_query, m, 0, false);
+    carve_vec(c, n_query, m, d, 0, false, precision);
     return c.used + 256;
 }
 
@@ -214,7 +348,7 @@ int nngp_predict(const double* x_train, const double* y_train, int64_t n_train, 
     int rc = to_kern_params(kernel, d, &kp);
     if (rc) return rc;
     Carver c(workspace, workspace_bytes);
-    VecLayout L = carve_vec(c, n_query, m, 0, false);
+    VecLayout L = carve_vec(c, n_query, m, d, 0, false, precision);
     if (!c.ok() || !workspace) return NNGP_ERR_WORKSPACE_TOO_SMALL;
     hipLaunchKernelGGL(zero_header_kernel, dim3(1), dim3(64), 0, st, L.hdr);
     VecArgs a{};
@@ -235,7 +369,8 @@ int nngp_predict(const double* x_train, const double* y_train, int64_t n_train, 
     a.mean = out_mean;
     a.stdv = out_std;
     a.hdr = L.hdr;
-    if (n_query > 0) launch_generic(a, L.wsL, L.wsv, L.P, st);
+    rc = run_fused(a, L, precision, st);
+    if (rc) return rc;
     NNGP_HIP_CHECK(hipGetLastError());
     return NNGP_OK;
 }

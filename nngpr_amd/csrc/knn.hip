@@ -97,45 +97,53 @@ __global__ void minmax_kernel(PrepArgs a, long long* mm) {
     }
 }
 
-// Writes, for points [0, n):
+// Index of feature f of point i in the blocked layout used by the scans: points
+// are grouped in blocks of 8, each block stores its nv feature rows contiguously
+// (f * 8 + lane), so one wave-uniform block is a few wide scalar loads.
+__host__ __device__ __forceinline__ int64_t blk8(int64_t i, int f, int nv) {
+    return (i >> 3) * (int64_t)(nv * 8) + f * 8 + (i & 7);
+}
+
+// Writes, for points [0, n) (entries up to npad are padding):
 //   u64  (n x d AoS): scaled FP64 coordinates (u = x / scale)
-//   s64  (d x npad SoA): same values, feature-major (exact scan); pad = +inf
-//   s32  ((d+1) x npad SoA, optional): fl32(u - centre) and, in row d, -|fl32(u - c)|^2
+//   s64  (blocked, nv = dp): same values for the exact scan; padding points +inf
+//   s32  (blocked, nv = dp + 1): fl32(u - centre), and -|fl32(u - centre)|^2 as
+//        feature dp; padding points have -inf there so they never pass a test
 //   p2   (n, optional): |u - centre|^2 in FP64
 //   smax (optional): max_i |u_i - centre| over these points (atomic, float bits)
-//   rows d .. dp-1 of s64/s32 are zero (padding features), the norm row is row dp.
+// Features d .. dp-1 are zero: they leave every key unchanged.
 __global__ void prep_kernel(PrepArgs a, const long long* mm, int64_t npad, int dp, double* u64,
                             double* s64, float* s32, double* p2, unsigned int* smax) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npad) return;
     if (i >= a.n) {
         for (int f = 0; f < dp; ++f) {
-            if (s64) s64[f * npad + i] = f == 0 ? __builtin_inf() : 0.0;
-            if (s32) s32[f * npad + i] = 0.0f;
+            if (s64) s64[blk8(i, f, dp)] = f == 0 ? __builtin_inf() : 0.0;
+            if (s32) s32[blk8(i, f, dp + 1)] = 0.0f;
         }
-        if (s32) s32[dp * npad + i] = -__builtin_inff();
+        if (s32) s32[blk8(i, dp, dp + 1)] = -__builtin_inff();
         return;
     }
     for (int f = a.d; f < dp; ++f) {
-        if (s64) s64[f * npad + i] = 0.0;
-        if (s32) s32[f * npad + i] = 0.0f;
+        if (s64) s64[blk8(i, f, dp)] = 0.0;
+        if (s32) s32[blk8(i, f, dp + 1)] = 0.0f;
     }
     double nrm = 0.0, pp = 0.0;
     for (int f = 0; f < a.d; ++f) {
         double v = scaled(a, i, f);
         if (u64) u64[i * a.d + f] = v;
-        if (s64) s64[f * npad + i] = v;
+        if (s64) s64[blk8(i, f, dp)] = v;
         if (s32) {
             double c = 0.5 * (key_dbl(mm[2 * f]) + key_dbl(mm[2 * f + 1]));
             double p = v - c;
             float pf = (float)p;
-            s32[f * npad + i] = pf;
+            s32[blk8(i, f, dp + 1)] = pf;
             nrm += (double)pf * (double)pf;
             pp += p * p;
         }
     }
     if (s32) {
-        s32[dp * npad + i] = -(float)nrm;
+        s32[blk8(i, dp, dp + 1)] = -(float)nrm;
         if (p2) p2[i] = pp;
         if (smax) {
             float r = (float)sqrt(fmax(nrm, pp)) * 1.0001f;
@@ -149,11 +157,9 @@ __global__ void prep_kernel(PrepArgs a, const long long* mm, int64_t npad, int d
 // ---------------------------------------------------------------------------
 template <typename T>
 struct ScanArgs {
-    const T* ref;        // SoA: rows 0..d-1 coordinates, (screen) row d = -|r|^2
-    int64_t ref_pad;
-    int64_t n_ref;
-    const T* qry;        // SoA coordinates, qry_pad columns
-    int64_t qry_pad;
+    const T* ref;        // blocked layout (blk8), nv = D (exact) or D + 1 (screen)
+    int32_t n_ref;
+    const T* qry;        // blocked layout, same nv
     int64_t n_query;
     int has_cut;
     int64_t cut;
@@ -163,30 +169,24 @@ struct ScanArgs {
     float* out_thr;      // screen: K-th candidate score per row
 };
 
-__device__ __forceinline__ int64_t wave_min64(int64_t v) {
+__device__ __forceinline__ int32_t wave_min32(int32_t v) {
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        int64_t o = __shfl_xor(v, off, 64);
-        v = o < v ? o : v;
-    }
-    return v;
+    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+    return __builtin_amdgcn_readfirstlane(v);
 }
-__device__ __forceinline__ int64_t wave_max64(int64_t v) {
+__device__ __forceinline__ int32_t wave_max32(int32_t v) {
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        int64_t o = __shfl_xor(v, off, 64);
-        v = o > v ? o : v;
-    }
-    return v;
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return __builtin_amdgcn_readfirstlane(v);
 }
 
 template <typename T, int K>
 __device__ __forceinline__ void topk_insert(T (&keys)[K], int (&ids)[K], T xk, int xi) {
-    // Compare-exchange ladder over a sorted register list.  New entries come in
-    // increasing ref index, so a strict '<' keeps equal keys in index order.
+    // Compare-exchange ladder over a sorted register list, ordered by (key, index):
+    // the carried element must move ahead of an equal key with a higher index.
 #pragma unroll
     for (int t = 0; t < K; ++t) {
-        bool lt = xk < keys[t];
+        bool lt = xk < keys[t] || (xk == keys[t] && (unsigned)xi < (unsigned)ids[t]);
         T nk = lt ? xk : keys[t];
         T ok = lt ? keys[t] : xk;
         int ni = lt ? xi : ids[t];
@@ -198,50 +198,96 @@ __device__ __forceinline__ void topk_insert(T (&keys)[K], int (&ids)[K], T xk, i
     }
 }
 
-template <typename T, int D>
-__device__ __forceinline__ T exact_score(const T* ref, int64_t pad, int64_t r, const T (&qr)[D]) {
-    T t0 = ref[r] - qr[0];
-    T acc = t0 * t0;
-#pragma unroll
-    for (int f = 1; f < D; ++f) {
-        T t = ref[f * pad + r] - qr[f];
-        acc = fma(t, t, acc);
-    }
-    return acc;
+// Order-preserving map of an FP32 score onto uint32, packed with the ref index:
+// comparing the packed uint64 values orders by (score, index) and puts empty
+// slots (+inf, index -1) last.
+__device__ __forceinline__ uint32_t ord_f32(float f) {
+    const uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float unord_f32(uint32_t u) {
+    return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+__device__ __forceinline__ uint64_t pack_key(float s, int idx) {
+    return ((uint64_t)ord_f32(s) << 32) | (uint32_t)idx;
 }
 
-template <typename T, int D, int K, int B, int U, bool SCREEN>
+template <int K>
+__device__ __forceinline__ void topk_insert_packed(uint64_t (&e)[K], uint64_t x) {
+#pragma unroll
+    for (int t = 0; t < K; ++t) {
+        const uint64_t lo = x < e[t] ? x : e[t];
+        x = x < e[t] ? e[t] : x;
+        e[t] = lo;
+    }
+}
+
+// Score of ref u of an 8-ref block for one lane's query.
+//   screen: -thr + sum_f (-2 q_f) r_f, compared against -|r|^2 (feature D);
+//   exact:  sum_f (r_f - q_f)^2 with the FMA order of the reference key.
+template <typename T, int D, bool SCREEN>
+__device__ __forceinline__ T block_score(const T* blk, int u, const T (&qr)[D], T negthr) {
+    if constexpr (SCREEN) {
+        T acc = negthr;
+#pragma unroll
+        for (int f = 0; f < D; ++f) acc = fma(qr[f], blk[f * 8 + u], acc);
+        return acc;
+    } else {
+        T t0 = blk[u] - qr[0];
+        T acc = t0 * t0;
+#pragma unroll
+        for (int f = 1; f < D; ++f) {
+            T t = blk[f * 8 + u] - qr[f];
+            acc = fma(t, t, acc);
+        }
+        return acc;
+    }
+}
+
+template <typename T, int D, int K, int B, bool SCREEN>
 __global__ __launch_bounds__(64) void scan_kernel(ScanArgs<T> a) {
+    constexpr int NV = SCREEN ? D + 1 : D;
     __shared__ T sbk[B][64];
     __shared__ int sbi[B][64];
     const int lane = threadIdx.x;
     const int64_t blk = a.reverse ? (int64_t)(gridDim.x - 1 - blockIdx.x) : (int64_t)blockIdx.x;
     const int64_t q = blk * 64 + lane;
     const bool qv = q < a.n_query;
-    int64_t lim = a.n_ref;
+    int64_t lim64 = a.n_ref;
     if (a.has_cut) {
-        lim = q + a.cut;
-        lim = lim < 0 ? 0 : (lim > a.n_ref ? a.n_ref : lim);
+        lim64 = q + a.cut;
+        lim64 = lim64 < 0 ? 0 : (lim64 > a.n_ref ? a.n_ref : lim64);
     }
-    if (!qv) lim = 0;
-    const int64_t lim_lo = wave_min64(qv ? lim : a.n_ref);
-    const int64_t lim_hi = wave_max64(lim);
+    const int32_t lim = qv ? (int32_t)lim64 : 0;
+    const int32_t lim_lo = wave_min32(qv ? lim : a.n_ref);
+    const int32_t lim_hi = wave_max32(lim);
+    const int32_t full_end = lim_lo & ~7;  // bulk region, whole 8-ref blocks
 
     T qr[D];
     const int64_t qc = qv ? q : 0;
 #pragma unroll
     for (int f = 0; f < D; ++f) {
-        T v = a.qry[f * a.qry_pad + qc];
+        T v = a.qry[blk8(qc, f, NV)];
         qr[f] = SCREEN ? (T)(-2) * v : v;
     }
-    T keys[K];
-    int ids[K];
+    // Sorted top-K list: (key, index) pairs (exact), or packed (score, index)
+    // uint64 values (screen).
+    constexpr int KE = SCREEN ? 1 : K;
+    constexpr int KP = SCREEN ? K : 1;
+    T keys[KE];
+    int ids[KE];
+    uint64_t pk[KP];
 #pragma unroll
-    for (int t = 0; t < K; ++t) {
+    for (int t = 0; t < KE; ++t) {
         keys[t] = dev_inf<T>();
         ids[t] = -1;
     }
+    // Empty slot: (+inf, index -1); its unpacked score is +inf, never NaN.
+#pragma unroll
+    for (int t = 0; t < KP; ++t) pk[t] = ((uint64_t)0xFF800000u << 32) | 0xFFFFFFFFu;
+    // Admission threshold: screen compares score - thr < -|r|^2, exact key < thr.
     T thr = qv ? dev_inf<T>() : -dev_inf<T>();
+    T negthr = -thr;
     int cnt = 0;
 
     auto merge = [&]() {
@@ -250,61 +296,109 @@ __global__ __launch_bounds__(64) void scan_kernel(ScanArgs<T> a) {
             if (b < cnt) {
                 T xk = sbk[b][lane];
                 int xi = sbi[b][lane];
-                if (xk < keys[K - 1]) topk_insert<T, K>(keys, ids, xk, xi);
+                if constexpr (SCREEN) {
+                    const uint64_t x = pack_key((float)xk, xi);
+                    if (x < pk[K - 1]) topk_insert_packed<K>(pk, x);
+                } else {
+                    if (xk < keys[K - 1]) topk_insert<T, K>(keys, ids, xk, xi);
+                }
             }
         }
         cnt = 0;
-        thr = qv ? keys[K - 1] : -dev_inf<T>();
+        if constexpr (SCREEN) {
+            thr = qv ? (T)unord_f32((uint32_t)(pk[K - 1] >> 32)) : -dev_inf<T>();
+        } else {
+            thr = qv ? keys[KE - 1] : -dev_inf<T>();
+        }
+        negthr = -thr;
     };
 
-    for (int64_t r0 = 0; r0 < lim_hi; r0 += U) {
-        const bool full = (r0 + U) <= lim_lo;
-        unsigned hits = 0u;
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int64_t r = r0 + u;
-            bool hit;
-            if constexpr (SCREEN) {
-                T acc = -thr;
-#pragma unroll
-                for (int f = 0; f < D; ++f) acc = fma(qr[f], a.ref[f * a.ref_pad + r], acc);
-                hit = acc < a.ref[D * a.ref_pad + r];
-            } else {
-                hit = exact_score<T, D>(a.ref, a.ref_pad, r, qr) < thr;
-            }
-            if (!full) hit = hit && (r < lim);
-            hits |= hit ? (1u << u) : 0u;
-        }
-        if (__any(hits != 0u)) {
+    // Appends the lanes' hits of one block (rare path), merging when a buffer fills.
+    auto take_block = [&](const T* bp, int32_t r0, int nref, bool masked) {
 #pragma unroll 1
-            for (int u = 0; u < U; ++u) {
-                if ((hits >> u) & 1u) {
-                    const int64_t r = r0 + u;
-                    T s;
-                    if constexpr (SCREEN) {
-                        s = -a.ref[D * a.ref_pad + r];
+        for (int u = 0; u < nref; ++u) {
+            const int32_t r = r0 + u;
+            T sc = block_score<T, D, SCREEN>(bp, u, qr, negthr);
+            bool hit = SCREEN ? (sc < bp[D * 8 + u]) : (sc < thr);
+            if (masked) hit = hit && (r < lim);
+            if (hit) {
+                T st;
+                if constexpr (SCREEN) {
+                    st = -bp[D * 8 + u];
 #pragma unroll
-                        for (int f = 0; f < D; ++f) s = fma(qr[f], a.ref[f * a.ref_pad + r], s);
-                    } else {
-                        s = exact_score<T, D>(a.ref, a.ref_pad, r, qr);
-                    }
-                    sbk[cnt][lane] = s;
-                    sbi[cnt][lane] = (int)r;
-                    ++cnt;
+                    for (int f = 0; f < D; ++f) st = fma(qr[f], bp[f * 8 + u], st);
+                } else {
+                    st = sc;
                 }
-                if (__any(cnt == B)) merge();
+                sbk[cnt][lane] = st;
+                sbi[cnt][lane] = r;
+                ++cnt;
+            }
+            if (__any(cnt == B)) merge();
+        }
+    };
+
+    // Bulk: every lane may take every ref.  Refs are read as sub-blocks of SUB
+    // refs (wave-uniform scalar loads), the next one prefetched while the current
+    // one is scored (the padded layout keeps the prefetch past full_end in bounds);
+    // the rare hit path re-reads them.
+    constexpr int SUB = (NV * sizeof(T) <= 16) ? 8 : 4;
+    T rv[NV][SUB];
+    {
+#pragma unroll
+        for (int f = 0; f < NV; ++f)
+#pragma unroll
+            for (int u = 0; u < SUB; ++u) rv[f][u] = a.ref[f * 8 + u];
+    }
+    for (int32_t r0 = 0; r0 < full_end; r0 += SUB) {
+        const T* bp = a.ref + (int64_t)(r0 >> 3) * (NV * 8) + (r0 & 7);
+        const int32_t r1 = r0 + SUB;
+        const T* bn = a.ref + (int64_t)(r1 >> 3) * (NV * 8) + (r1 & 7);
+        T rn[NV][SUB];
+#pragma unroll
+        for (int f = 0; f < NV; ++f)
+#pragma unroll
+            for (int u = 0; u < SUB; ++u) rn[f][u] = bn[f * 8 + u];
+        bool any = false;
+#pragma unroll
+        for (int u = 0; u < SUB; ++u) {
+            T sc;
+            if constexpr (SCREEN) {
+                sc = negthr;
+#pragma unroll
+                for (int f = 0; f < D; ++f) sc = fma(qr[f], rv[f][u], sc);
+                any = any | (sc < rv[D][u]);
+            } else {
+                T t0 = rv[0][u] - qr[0];
+                sc = t0 * t0;
+#pragma unroll
+                for (int f = 1; f < D; ++f) {
+                    T t = rv[f][u] - qr[f];
+                    sc = fma(t, t, sc);
+                }
+                any = any | (sc < thr);
             }
         }
+        if (__builtin_amdgcn_ballot_w64(any)) take_block(bp, r0, SUB, false);
+#pragma unroll
+        for (int f = 0; f < NV; ++f)
+#pragma unroll
+            for (int u = 0; u < SUB; ++u) rv[f][u] = rn[f][u];
+    }
+    // Tail: refs past some lanes' Vecchia limit.
+    for (int32_t r0 = full_end & ~7; r0 < lim_hi; r0 += 8) {
+        const T* bp = a.ref + (int64_t)(r0 >> 3) * (NV * 8);
+        take_block(bp, r0, 8, true);
     }
     if (__any(cnt > 0)) merge();
     if (!qv) return;
     if constexpr (SCREEN) {
 #pragma unroll
-        for (int t = 0; t < K; ++t) a.out_idx[q * K + t] = ids[t];
-        a.out_thr[q] = (float)keys[K - 1];
+        for (int t = 0; t < K; ++t) a.out_idx[q * K + t] = (int)(uint32_t)pk[t];
+        a.out_thr[q] = unord_f32((uint32_t)(pk[K - 1] >> 32));
     } else {
 #pragma unroll
-        for (int t = 0; t < K; ++t)
+        for (int t = 0; t < KE; ++t)
             if (t < a.k_out) a.out_idx[q * a.k_out + t] = ids[t];
     }
 }
@@ -537,11 +631,9 @@ KnnLayout carve_knn(void* ws, size_t bytes, int64_t n_ref, int64_t n_query, int 
 
 template <typename T, int D, int K, bool SCREEN>
 void launch_scan_u(const ScanArgs<T>& a, hipStream_t st) {
-    constexpr int U = (D <= 4) ? 8 : 4;
     constexpr int B = 16;
     const int64_t nblk = (a.n_query + 63) / 64;
-    hipLaunchKernelGGL((scan_kernel<T, D, K, B, U, SCREEN>), dim3((unsigned)nblk), dim3(64), 0,
-                       st, a);
+    hipLaunchKernelGGL((scan_kernel<T, D, K, B, SCREEN>), dim3((unsigned)nblk), dim3(64), 0, st, a);
 }
 
 template <typename T, int D, bool SCREEN>
@@ -679,17 +771,18 @@ int knn_run(const double* ref, int64_t n_ref, const double* query, int64_t n_que
     if (!screen) {
         ScanArgs<double> a{};
         a.ref = L.rs64;
-        a.ref_pad = rpad;
-        a.n_ref = n_ref;
+        a.n_ref = (int32_t)n_ref;
         a.qry = qs64;
-        a.qry_pad = same ? rpad : qpad;
         a.n_query = n_query;
         a.has_cut = has_cut;
         a.cut = ref_cut;
         a.reverse = has_cut;
         a.k_out = k;
         a.out_idx = out_idx;
-        if (launch_scan<double, false>(a, d, exact_K(k), st) != 0) return NNGP_ERR_UNSUPPORTED;
+        timing_begin(TK_KNN_SCAN, st);
+        const int rc = launch_scan<double, false>(a, d, exact_K(k), st);
+        timing_end(TK_KNN_SCAN, st);
+        if (rc != 0) return NNGP_ERR_UNSUPPORTED;
         NNGP_HIP_CHECK(hipGetLastError());
         return NNGP_OK;
     }
@@ -697,10 +790,8 @@ int knn_run(const double* ref, int64_t n_ref, const double* query, int64_t n_que
     const int K = screen_K(k);
     ScanArgs<float> a{};
     a.ref = L.rs32;
-    a.ref_pad = rpad;
-    a.n_ref = n_ref;
+    a.n_ref = (int32_t)n_ref;
     a.qry = qs32;
-    a.qry_pad = same ? rpad : qpad;
     a.n_query = n_query;
     a.has_cut = has_cut;
     a.cut = ref_cut;
@@ -708,7 +799,10 @@ int knn_run(const double* ref, int64_t n_ref, const double* query, int64_t n_que
     a.out_idx = L.cand;
     a.out_thr = L.thr;
     hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, L.flag_count, 1);
-    if (launch_scan<float, true>(a, d, K, st) != 0) return NNGP_ERR_UNSUPPORTED;
+    timing_begin(TK_KNN_SCAN, st);
+    const int rcs = launch_scan<float, true>(a, d, K, st);
+    timing_end(TK_KNN_SCAN, st);
+    if (rcs != 0) return NNGP_ERR_UNSUPPORTED;
 
     RerankArgs rr{};
     rr.ru64 = L.ru64;
@@ -724,7 +818,10 @@ int knn_run(const double* ref, int64_t n_ref, const double* query, int64_t n_que
     rr.flag_rows = L.flag_rows;
     rr.flag_count = L.flag_count;
     rr.eps_factor = (4.0 * d + 16.0) * std::ldexp(1.0, -24);
-    if (launch_rerank(rr, K, st) != 0) return NNGP_ERR_UNSUPPORTED;
+    timing_begin(TK_KNN_RERANK, st);
+    const int rcr = launch_rerank(rr, K, st);
+    timing_end(TK_KNN_RERANK, st);
+    if (rcr != 0) return NNGP_ERR_UNSUPPORTED;
 
     SimpleArgs sa{};
     sa.ru64 = L.ru64;

@@ -1,0 +1,18 @@
+// Internal interface of the register-resident fused Vecchia kernel (m <= 64).
+#pragma once
+#include "vecchia.h"
+
+namespace nngp {
+
+struct RegExtra {
+    const int64_t* plist;                 // optional indirection: slot -> point
+    const unsigned long long* pcount;     // optional device count of slots
+    int64_t* flag_list;                   // FP32 form: points to redo in FP64
+    unsigned long long* flag_count;
+};
+
+int reg_mr(int m);
+size_t reg_lds_bytes(int m, int d, bool fp32);
+int launch_reg(const VecArgs& a, const RegExtra& ex, bool fp32, int64_t max_points, hipStream_t st);
+
+}  // namespace nngp
