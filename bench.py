@@ -150,6 +150,7 @@ def main():
         t64 = timed(lambda: evaluate(F64), max(2, args.steps // 2), 1)
         ll64, g64, nn64 = evaluate(F64)
         ll32, g32, nn32 = evaluate(F32)
+        knn_stats = hip_ops.knn_stats(dev=dev)
         knn_exact = bool(torch.equal(nn64, nn32))
         t32c = timed(lambda: evaluate(F32, nn32), args.steps, 1)
         t64c = timed(lambda: evaluate(F64, nn64), args.steps, 1)
@@ -185,6 +186,7 @@ def main():
                 "std_rel_max": float(rel_s.max()), "std_rel_mean": float(rel_s.mean()),
             },
             "knn_indices_identical_fp32_vs_fp64": knn_exact,
+            "knn_rows_recomputed_fp32": knn_stats["knn_rows_recomputed"] if knn_stats else None,
             "stage_ms_fp32": {"knn_scan": scan_avg, "knn_rerank": rerank_ms / max(scan_n, 1),
                               "fused_loglik_grad": fused_avg},
             "stage_ms_fp64": {"fused_loglik_grad": f64_ms},

@@ -214,10 +214,13 @@ __device__ __forceinline__ uint64_t pack_key(float s, int idx) {
 
 template <int K>
 __device__ __forceinline__ void topk_insert_packed(uint64_t (&e)[K], uint64_t x) {
+    // Screening needs the candidate set, not an order among equal scores: only
+    // the score half is compared (one 32-bit compare and four selects per step).
 #pragma unroll
     for (int t = 0; t < K; ++t) {
-        const uint64_t lo = x < e[t] ? x : e[t];
-        x = x < e[t] ? e[t] : x;
+        const bool lt = (uint32_t)(x >> 32) < (uint32_t)(e[t] >> 32);
+        const uint64_t lo = lt ? x : e[t];
+        x = lt ? e[t] : x;
         e[t] = lo;
     }
 }
@@ -313,10 +316,12 @@ __global__ __launch_bounds__(64) void scan_kernel(ScanArgs<T> a) {
         negthr = -thr;
     };
 
-    // Appends the lanes' hits of one block (rare path), merging when a buffer fills.
+    // Appends the lanes' hits of one block (rare path); the buffer is merged
+    // before an append could overflow it.
     auto take_block = [&](const T* bp, int32_t r0, int nref, bool masked) {
 #pragma unroll 1
         for (int u = 0; u < nref; ++u) {
+            if (__any(cnt >= B)) merge();
             const int32_t r = r0 + u;
             T sc = block_score<T, D, SCREEN>(bp, u, qr, negthr);
             bool hit = SCREEN ? (sc < bp[D * 8 + u]) : (sc < thr);
@@ -334,56 +339,80 @@ __global__ __launch_bounds__(64) void scan_kernel(ScanArgs<T> a) {
                 sbi[cnt][lane] = r;
                 ++cnt;
             }
-            if (__any(cnt == B)) merge();
         }
     };
 
     // Bulk: every lane may take every ref.  Refs are read as sub-blocks of SUB
-    // refs (wave-uniform scalar loads), the next one prefetched while the current
-    // one is scored (the padded layout keeps the prefetch past full_end in bounds);
-    // the rare hit path re-reads them.
+    // refs through wave-uniform scalar loads into two ping-pong register sets,
+    // the next sub-block prefetched while the current one is scored (the padded
+    // layout keeps a prefetch past full_end in bounds).  The per-ref test value
+    // A sub-block with no hit in any lane costs its scores and one branch.
     constexpr int SUB = (NV * sizeof(T) <= 16) ? 8 : 4;
-    T rv[NV][SUB];
-    {
+    auto sub_ptr = [&](int32_t r) { return a.ref + (int64_t)(r >> 3) * (NV * 8) + (r & 7); };
+    auto load_sub = [&](T (&rv)[NV][SUB], const T* bp) {
 #pragma unroll
         for (int f = 0; f < NV; ++f)
 #pragma unroll
-            for (int u = 0; u < SUB; ++u) rv[f][u] = a.ref[f * 8 + u];
-    }
-    for (int32_t r0 = 0; r0 < full_end; r0 += SUB) {
-        const T* bp = a.ref + (int64_t)(r0 >> 3) * (NV * 8) + (r0 & 7);
-        const int32_t r1 = r0 + SUB;
-        const T* bn = a.ref + (int64_t)(r1 >> 3) * (NV * 8) + (r1 & 7);
-        T rn[NV][SUB];
-#pragma unroll
-        for (int f = 0; f < NV; ++f)
-#pragma unroll
-            for (int u = 0; u < SUB; ++u) rn[f][u] = bn[f * 8 + u];
+            for (int u = 0; u < SUB; ++u) rv[f][u] = bp[f * 8 + u];
+    };
+    auto score_sub = [&](const T (&rv)[NV][SUB], int32_t r0) {
+        bool hm[SUB];
+        T kv[SUB];
         bool any = false;
 #pragma unroll
         for (int u = 0; u < SUB; ++u) {
-            T sc;
             if constexpr (SCREEN) {
-                sc = negthr;
+                T acc = negthr;
 #pragma unroll
-                for (int f = 0; f < D; ++f) sc = fma(qr[f], rv[f][u], sc);
-                any = any | (sc < rv[D][u]);
+                for (int f = 0; f < D; ++f) acc = fma(qr[f], rv[f][u], acc);
+                hm[u] = acc < rv[D][u];
             } else {
                 T t0 = rv[0][u] - qr[0];
-                sc = t0 * t0;
+                T acc = t0 * t0;
 #pragma unroll
                 for (int f = 1; f < D; ++f) {
                     T t = rv[f][u] - qr[f];
-                    sc = fma(t, t, sc);
+                    acc = fma(t, t, acc);
                 }
-                any = any | (sc < thr);
+                kv[u] = acc;
+                hm[u] = acc < thr;
+            }
+            any = any | hm[u];
+        }
+        if (__builtin_amdgcn_ballot_w64(any)) {
+            // Room for this sub-block's hits: at most SUB per lane.
+            if (__any(cnt > B - SUB)) merge();
+#pragma unroll
+            for (int u = 0; u < SUB; ++u) {
+                if (hm[u]) {
+                    T st;
+                    if constexpr (SCREEN) {
+                        st = -rv[D][u];
+#pragma unroll
+                        for (int f = 0; f < D; ++f) st = fma(qr[f], rv[f][u], st);
+                    } else {
+                        st = kv[u];
+                    }
+                    sbk[cnt][lane] = st;
+                    sbi[cnt][lane] = r0 + u;
+                    ++cnt;
+                }
             }
         }
-        if (__builtin_amdgcn_ballot_w64(any)) take_block(bp, r0, SUB, false);
-#pragma unroll
-        for (int f = 0; f < NV; ++f)
-#pragma unroll
-            for (int u = 0; u < SUB; ++u) rv[f][u] = rn[f][u];
+    };
+    {
+        T ra[NV][SUB], rb[NV][SUB];
+        load_sub(ra, a.ref);
+        int32_t r0 = 0;
+        while (r0 < full_end) {
+            load_sub(rb, sub_ptr(r0 + SUB));
+            score_sub(ra, r0);
+            r0 += SUB;
+            if (r0 >= full_end) break;
+            load_sub(ra, sub_ptr(r0 + SUB));
+            score_sub(rb, r0);
+            r0 += SUB;
+        }
     }
     // Tail: refs past some lanes' Vecchia limit.
     for (int32_t r0 = full_end & ~7; r0 < lim_hi; r0 += 8) {
@@ -419,6 +448,8 @@ struct RerankArgs {
     int32_t* out_idx;     // n_query x k
     int64_t* flag_rows;   // flagged rows list
     unsigned long long* flag_count;
+    double* flag_key;     // per flagged slot: (key, index) of the m-th candidate,
+    int32_t* flag_idx;    //   an upper bound of the row's true m-th pair
     double eps_factor;    // (4 d + 16) * 2^-24
 };
 
@@ -452,6 +483,7 @@ __global__ __launch_bounds__(256) void rerank_kernel(RerankArgs a) {
         }
     }
     double dm = -__builtin_inf();
+    int dmi = -1;
 #pragma unroll
     for (int t = 0; t < K; ++t) {
         if (ids[t] < 0) continue;
@@ -463,7 +495,10 @@ __global__ __launch_bounds__(256) void rerank_kernel(RerankArgs a) {
             rank += before ? 1 : 0;
         }
         if (rank < a.k) a.out_idx[q * a.k + rank] = ids[t];
-        if (rank == a.k - 1) dm = key[t];
+        if (rank == a.k - 1) {
+            dm = key[t];
+            dmi = ids[t];
+        }
     }
     for (int j = count; j < a.k; ++j) a.out_idx[q * a.k + j] = -1;
     if (count < K) return;  // every ref of the row was a candidate: exact
@@ -476,6 +511,93 @@ __global__ __launch_bounds__(256) void rerank_kernel(RerankArgs a) {
     if (!(dm < bound)) {
         unsigned long long slot = atomicAdd(a.flag_count, 1ULL);
         a.flag_rows[slot] = q;
+        a.flag_key[slot] = dm;
+        a.flag_idx[slot] = dmi;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Wave-parallel exact recomputation of flagged rows.  One wave per row: lanes
+// stream the row's refs, keep every (key, index) <= the row's bound pair in LDS,
+// then rank the kept entries.  The true top-k pairs are all <= the bound, so they
+// are all kept.  Rows that keep more than CAP entries (massive exact ties) go to
+// the per-row serial kernel instead.
+// ---------------------------------------------------------------------------
+struct WaveFbArgs {
+    const double* ru64;
+    const double* qu64;
+    int d;
+    int64_t n_ref;
+    int has_cut;
+    int64_t cut;
+    int k;
+    const int64_t* rows;
+    const unsigned long long* row_count;
+    const double* bound_key;
+    const int32_t* bound_idx;
+    int32_t* out_idx;
+    int64_t* over_rows;                 // rows handed to the serial kernel
+    unsigned long long* over_count;
+};
+
+constexpr int kFbCap = 1024;
+constexpr int kFbWaves = 4;
+
+__global__ __launch_bounds__(256) void wave_fallback_kernel(WaveFbArgs a) {
+    __shared__ double skey[kFbWaves][kFbCap];
+    __shared__ int sidx[kFbWaves][kFbCap];
+    __shared__ int scnt[kFbWaves];
+    const int w = threadIdx.x / 64;
+    const int lane = threadIdx.x % 64;
+    const int64_t nrows = (int64_t)*a.row_count;
+    for (int64_t slot = (int64_t)blockIdx.x * kFbWaves + w; slot < nrows;
+         slot += (int64_t)gridDim.x * kFbWaves) {
+        const int64_t q = a.rows[slot];
+        const double bk = a.bound_key[slot];
+        const int bi = a.bound_idx[slot];
+        int64_t lim = a.n_ref;
+        if (a.has_cut) {
+            lim = q + a.cut;
+            lim = lim < 0 ? 0 : (lim > a.n_ref ? a.n_ref : lim);
+        }
+        if (lane == 0) scnt[w] = 0;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        const double* uq = a.qu64 + q * a.d;
+        for (int64_t r = lane; r < lim; r += 64) {
+            const double key = exact_key(a.ru64 + r * a.d, uq, a.d);
+            if (key < bk || (key == bk && (int)r <= bi)) {
+                const int pos = atomicAdd(&scnt[w], 1);
+                if (pos < kFbCap) {
+                    skey[w][pos] = key;
+                    sidx[w][pos] = (int)r;
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        const int cnt = scnt[w];
+        if (cnt > kFbCap) {
+            if (lane == 0) {
+                unsigned long long o = atomicAdd(a.over_count, 1ULL);
+                a.over_rows[o] = q;
+            }
+        } else {
+            for (int e = lane; e < cnt; e += 64) {
+                const double ke = skey[w][e];
+                const int ie = sidx[w][e];
+                int rank = 0;
+                for (int f = 0; f < cnt; ++f) {
+                    const double kf = skey[w][f];
+                    const int ifx = sidx[w][f];
+                    rank += (kf < ke || (kf == ke && ifx < ie)) ? 1 : 0;
+                }
+                if (rank < a.k) a.out_idx[q * a.k + rank] = ie;
+            }
+            for (int j = cnt + lane; j < a.k; j += 64) a.out_idx[q * a.k + j] = -1;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -555,11 +677,9 @@ __global__ void fill_i32_kernel(int32_t* p, int64_t n, int32_t v) {
 // ---------------------------------------------------------------------------
 inline int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
-// Screen candidate count for k neighbours (multiple of 16, <= 80).
-inline int screen_K(int k) {
-    int K = (int)round_up(k + 16, 16);
-    return K;
-}
+// Screen candidate count for k neighbours: k plus a margin of 8, rounded to 8.
+// Rows whose margin turns out too thin fail the certificate and are redone.
+inline int screen_K(int k) { return (int)round_up(k + 8, 8); }
 
 struct KnnLayout {
     WsHeader* hdr;
@@ -577,6 +697,10 @@ struct KnnLayout {
     int32_t* cand;
     float* thr;
     int64_t* flag_rows;
+    double* flag_key;
+    int32_t* flag_idx;
+    int64_t* over_rows;
+    unsigned long long* over_count;
     double* lkey;
     int32_t* lidx;
 };
@@ -621,6 +745,10 @@ KnnLayout carve_knn(void* ws, size_t bytes, int64_t n_ref, int64_t n_query, int 
         L.cand = c.take<int32_t>((size_t)n_query * K + 1);
         L.thr = c.take<float>((size_t)n_query + 1);
         L.flag_rows = c.take<int64_t>((size_t)n_query + 1);
+        L.flag_key = c.take<double>((size_t)n_query + 1);
+        L.flag_idx = c.take<int32_t>((size_t)n_query + 1);
+        L.over_rows = c.take<int64_t>((size_t)n_query + 1);
+        L.over_count = c.take<unsigned long long>(4);
     }
     L.lkey = c.take<double>((size_t)k * kFallbackThreads);
     L.lidx = c.take<int32_t>((size_t)k * kFallbackThreads);
@@ -631,7 +759,7 @@ KnnLayout carve_knn(void* ws, size_t bytes, int64_t n_ref, int64_t n_query, int 
 
 template <typename T, int D, int K, bool SCREEN>
 void launch_scan_u(const ScanArgs<T>& a, hipStream_t st) {
-    constexpr int B = 16;
+    constexpr int B = SCREEN ? 32 : 16;
     const int64_t nblk = (a.n_query + 63) / 64;
     hipLaunchKernelGGL((scan_kernel<T, D, K, B, SCREEN>), dim3((unsigned)nblk), dim3(64), 0, st, a);
 }
@@ -640,10 +768,14 @@ template <typename T, int D, bool SCREEN>
 int launch_scan_d(const ScanArgs<T>& a, int K, hipStream_t st) {
     if constexpr (SCREEN) {
         switch (K) {
+            case 16: launch_scan_u<T, D, 16, SCREEN>(a, st); return 0;
+            case 24: launch_scan_u<T, D, 24, SCREEN>(a, st); return 0;
             case 32: launch_scan_u<T, D, 32, SCREEN>(a, st); return 0;
+            case 40: launch_scan_u<T, D, 40, SCREEN>(a, st); return 0;
             case 48: launch_scan_u<T, D, 48, SCREEN>(a, st); return 0;
+            case 56: launch_scan_u<T, D, 56, SCREEN>(a, st); return 0;
             case 64: launch_scan_u<T, D, 64, SCREEN>(a, st); return 0;
-            case 80: launch_scan_u<T, D, 80, SCREEN>(a, st); return 0;
+            case 72: launch_scan_u<T, D, 72, SCREEN>(a, st); return 0;
             default: return -1;
         }
     } else {
@@ -666,13 +798,22 @@ int launch_scan(const ScanArgs<T>& a, int d, int K, hipStream_t st) {
     }
 }
 
-int launch_rerank(const RerankArgs& a, int K, hipStream_t st) {
+template <int K>
+void launch_rerank_k(const RerankArgs& a, hipStream_t st) {
     const unsigned grid = (unsigned)((a.n_query + 255) / 256);
+    hipLaunchKernelGGL(rerank_kernel<K>, dim3(grid), dim3(256), 0, st, a);
+}
+
+int launch_rerank(const RerankArgs& a, int K, hipStream_t st) {
     switch (K) {
-        case 32: hipLaunchKernelGGL(rerank_kernel<32>, dim3(grid), dim3(256), 0, st, a); return 0;
-        case 48: hipLaunchKernelGGL(rerank_kernel<48>, dim3(grid), dim3(256), 0, st, a); return 0;
-        case 64: hipLaunchKernelGGL(rerank_kernel<64>, dim3(grid), dim3(256), 0, st, a); return 0;
-        case 80: hipLaunchKernelGGL(rerank_kernel<80>, dim3(grid), dim3(256), 0, st, a); return 0;
+        case 16: launch_rerank_k<16>(a, st); return 0;
+        case 24: launch_rerank_k<24>(a, st); return 0;
+        case 32: launch_rerank_k<32>(a, st); return 0;
+        case 40: launch_rerank_k<40>(a, st); return 0;
+        case 48: launch_rerank_k<48>(a, st); return 0;
+        case 56: launch_rerank_k<56>(a, st); return 0;
+        case 64: launch_rerank_k<64>(a, st); return 0;
+        case 72: launch_rerank_k<72>(a, st); return 0;
         default: return -1;
     }
 }
@@ -817,11 +958,31 @@ int knn_run(const double* ref, int64_t n_ref, const double* query, int64_t n_que
     rr.out_idx = out_idx;
     rr.flag_rows = L.flag_rows;
     rr.flag_count = L.flag_count;
+    rr.flag_key = L.flag_key;
+    rr.flag_idx = L.flag_idx;
     rr.eps_factor = (4.0 * d + 16.0) * std::ldexp(1.0, -24);
     timing_begin(TK_KNN_RERANK, st);
     const int rcr = launch_rerank(rr, K, st);
     timing_end(TK_KNN_RERANK, st);
     if (rcr != 0) return NNGP_ERR_UNSUPPORTED;
+
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, L.over_count, 1);
+    WaveFbArgs wf{};
+    wf.ru64 = L.ru64;
+    wf.qu64 = qu64;
+    wf.d = d;
+    wf.n_ref = n_ref;
+    wf.has_cut = has_cut;
+    wf.cut = ref_cut;
+    wf.k = k;
+    wf.rows = L.flag_rows;
+    wf.row_count = L.flag_count;
+    wf.bound_key = L.flag_key;
+    wf.bound_idx = L.flag_idx;
+    wf.out_idx = out_idx;
+    wf.over_rows = L.over_rows;
+    wf.over_count = L.over_count;
+    hipLaunchKernelGGL(wave_fallback_kernel, dim3(2048), dim3(64 * kFbWaves), 0, st, wf);
 
     SimpleArgs sa{};
     sa.ru64 = L.ru64;
@@ -832,8 +993,8 @@ int knn_run(const double* ref, int64_t n_ref, const double* query, int64_t n_que
     sa.has_cut = has_cut;
     sa.cut = ref_cut;
     sa.k = k;
-    sa.rows = L.flag_rows;
-    sa.row_count = L.flag_count;
+    sa.rows = L.over_rows;
+    sa.row_count = L.over_count;
     sa.lkey = L.lkey;
     sa.lidx = L.lidx;
     sa.out_idx = out_idx;
