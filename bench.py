@@ -62,6 +62,51 @@ def cpu_baseline(ds):
     }
 
 
+def serve_sweep(ds, rates, seconds=3.0, max_requests=300):
+    """Poisson request streams against the resident model: batch sizes
+    log-uniform in [1, 65536], queries drawn from the test points; returns p50/p99
+    latency (submit -> result) and points/s per arrival rate."""
+    import numpy as np
+    from nngpr_amd import hip_ops
+    from nngpr_amd.serving import NngpServer, ResidentModel
+    model = ResidentModel(ds["x_train"], ds["y_train"], ds["kernel"], alpha=ds["alpha"],
+                          num_nn=ds["num_nn"], precision=hip_ops.PRECISION_FP32_REFINED)
+    pool = ds["x_test"]
+    rng = np.random.default_rng(5)
+    out = {}
+    with NngpServer(model, max_batch_points=65536, max_slots=3) as srv:
+        for f in [srv.submit(pool[:5000]) for _ in range(3)]:
+            f.result()
+        for lam in rates:
+            n = int(min(max_requests, lam * seconds))
+            gaps = rng.exponential(1.0 / lam, size=n)
+            sizes = np.maximum(1, np.floor(np.exp(rng.uniform(0, np.log(65536), size=n)))).astype(int)
+            starts = rng.integers(0, pool.shape[0], size=n)
+            srv.completed.clear()
+            futs = []
+            t0 = time.perf_counter()
+            t_next = t0
+            for i in range(n):
+                t_next += gaps[i]
+                dt = t_next - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                idx = (starts[i] + np.arange(sizes[i])) % pool.shape[0]
+                futs.append(srv.submit(pool[idx]))
+            for f in futs:
+                f.result()
+            t1 = max(r.t_done for r in srv.completed)
+            lat = srv.latencies_ms()
+            out[str(lam)] = {"requests": n, "points": int(sizes.sum()),
+                             "p50_ms": float(np.percentile(lat, 50)),
+                             "p99_ms": float(np.percentile(lat, 99)),
+                             "points_per_s": float(sizes.sum() / (t1 - t0))}
+        out["slots"] = len(srv.slots)
+        out["slot_capacity"] = srv.slot_capacity
+    model.close()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -193,6 +238,10 @@ def main():
             "loglik_fp64": ll64.item(),
         }
 
+    serving = None
+    if not args.skip_extras:
+        serving = serve_sweep(ds, (50, 200, 1000))
+
     line = {
         "metric": "loglik+grad evals/s (FP32+FP64 refinement form, kNN recomputed per eval)",
         "value": 1.0 / step_s,
@@ -212,6 +261,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         **extras,
+        "serving": serving,
     }
     print(json.dumps(line))
 

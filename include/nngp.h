@@ -145,6 +145,29 @@ int nngp_predict(const double* x_train, const double* y_train, int64_t n_train, 
                  double* out_std, void* workspace, size_t workspace_bytes, void* stream);
 
 /*
+ * Resident prediction model for serving.  No single reference counterpart: it
+ * keeps the state the reference rebuilds on every predict call
+ * (_find_nn_indices_for_train_and_eval + _predict_worker, base_nngpr.py:1370-1464,
+ * 1032-1135) resident on the device: the training set, the kNN ref-side prepared
+ * coordinates/norms and the kernel parameters.  nngp_model_create copies
+ * x_train/y_train (device pointers), allocates the model's own device memory and
+ * synchronises `stream`; nngp_model_destroy frees it.  nngp_model_predict is
+ * nngp_knn (query side only, no cut, k = min(num_nn, n_train)) followed by
+ * nngp_predict, entirely on `stream`, with no allocation, no host
+ * synchronisation and no memset nodes: it may be captured into a HIP graph.
+ * Concurrent predicts on one model need distinct workspaces.
+ */
+typedef struct nngp_model nngp_model;
+int nngp_model_create(const double* x_train, const double* y_train, int64_t n_train, int d, int ny,
+                      const nngp_kernel* kernel, double alpha, int num_nn, const double* scale,
+                      int n_scale, int precision, nngp_model** out_model, void* stream);
+int nngp_model_destroy(nngp_model* model);
+size_t nngp_model_predict_workspace_bytes(const nngp_model* model, int64_t max_query);
+int nngp_model_predict(const nngp_model* model, const double* x_query, int64_t n_query,
+                       double* out_mean, double* out_std, void* workspace, size_t workspace_bytes,
+                       void* stream);
+
+/*
  * Counters of the last nngp_knn / nngp_vecchia_loglik / nngp_predict call that
  * used this workspace: rows whose kNN certificate failed and were recomputed
  * exactly, points re-factored in FP64 after the FP32 pivot check.  Synchronises

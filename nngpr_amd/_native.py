@@ -67,6 +67,11 @@ SIGNATURES = [
     ("nngp_predict", _I, [_P, _P, _I64, _I, _I, _P, _I64, _P, _I,
                           ctypes.POINTER(NngpKernel), _D, _I, _P, _P, _P, _SZ, _P]),
     ("nngp_read_stats", _I, [_P, ctypes.POINTER(NngpStats), _P]),
+    ("nngp_model_create", _I, [_P, _P, _I64, _I, _I, ctypes.POINTER(NngpKernel), _D, _I, _P, _I, _I,
+                               ctypes.POINTER(ctypes.c_void_p), _P]),
+    ("nngp_model_destroy", _I, [_P]),
+    ("nngp_model_predict_workspace_bytes", _SZ, [_P, _I64]),
+    ("nngp_model_predict", _I, [_P, _P, _I64, _P, _P, _P, _SZ, _P]),
     ("nngp_timing_enable", _I, [_I]),
     ("nngp_timing_read", _I, [_I, _I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I64)]),
 ]

@@ -187,6 +187,21 @@ __global__ void extract_col_kernel(const double* pt, int stride, int64_t n, doub
 }  // namespace
 }  // namespace nngp
 
+// Resident prediction model (opaque to callers): copies of the training set, the
+// kNN ref-side prepared state and the kernel parameters, in one device allocation.
+struct nngp_model {
+    int device;
+    int64_t n;
+    int d, ny, m, precision;
+    double alpha;
+    nngp::KernParams kp;
+    double* x;
+    double* y;
+    nngp::KnnRefs refs;
+    void* mem;
+    size_t mem_bytes;
+};
+
 using namespace nngp;
 
 extern "C" {
@@ -370,6 +385,135 @@ int nngp_predict(const double* x_train, const double* y_train, int64_t n_train, 
     a.stdv = out_std;
     a.hdr = L.hdr;
     rc = run_fused(a, L, precision, st);
+    if (rc) return rc;
+    NNGP_HIP_CHECK(hipGetLastError());
+    return NNGP_OK;
+}
+
+int nngp_model_create(const double* x_train, const double* y_train, int64_t n_train, int d, int ny,
+                      const nngp_kernel* kernel, double alpha, int num_nn, const double* scale,
+                      int n_scale, int precision, nngp_model** out_model, void* stream) {
+    hipStream_t st = (hipStream_t)stream;
+    if (!out_model || !x_train || !y_train || n_train < 1 || d < 1 || d > NNGP_MAX_DIM || ny < 1 ||
+        num_nn < 1)
+        return NNGP_ERR_INVALID_ARGUMENT;
+    if (precision != NNGP_PRECISION_FP64 && precision != NNGP_PRECISION_FP32_REFINED)
+        return NNGP_ERR_INVALID_ARGUMENT;
+    KernParams kp;
+    int rc = to_kern_params(kernel, d, &kp);
+    if (rc) return rc;
+    const int m = (int)(num_nn < n_train ? num_nn : n_train);
+    if (m > NNGP_MAX_NN) return NNGP_ERR_UNSUPPORTED;
+    Carver c(nullptr, 0);
+    c.take<double>((size_t)n_train * d);
+    c.take<double>((size_t)n_train * ny);
+    const size_t rb = knn_refs_bytes(n_train, d, precision);
+    c.take<char>(rb);
+    nngp_model* M = new nngp_model();
+    M->mem_bytes = c.used + 256;
+    if (hipMalloc(&M->mem, M->mem_bytes) != hipSuccess) {
+        delete M;
+        return NNGP_ERR_HIP;
+    }
+    Carver cm(M->mem, M->mem_bytes);
+    M->x = cm.take<double>((size_t)n_train * d);
+    M->y = cm.take<double>((size_t)n_train * ny);
+    char* rmem = cm.take<char>(rb);
+    hipGetDevice(&M->device);
+    M->n = n_train;
+    M->d = d;
+    M->ny = ny;
+    M->m = m;
+    M->precision = precision;
+    M->alpha = alpha;
+    M->kp = kp;
+    hipError_t e1 = hipMemcpyAsync(M->x, x_train, sizeof(double) * n_train * d,
+                                   hipMemcpyDeviceToDevice, st);
+    hipError_t e2 = hipMemcpyAsync(M->y, y_train, sizeof(double) * n_train * ny,
+                                   hipMemcpyDeviceToDevice, st);
+    if (e1 != hipSuccess || e2 != hipSuccess) {
+        hipFree(M->mem);
+        delete M;
+        return NNGP_ERR_HIP;
+    }
+    rc = knn_prepare_refs(M->x, n_train, d, scale, n_scale, precision, rmem, rb, &M->refs, st);
+    if (rc == NNGP_OK && hipStreamSynchronize(st) != hipSuccess) rc = NNGP_ERR_HIP;
+    if (rc) {
+        hipFree(M->mem);
+        delete M;
+        return rc;
+    }
+    *out_model = M;
+    return NNGP_OK;
+}
+
+int nngp_model_destroy(nngp_model* model) {
+    if (!model) return NNGP_ERR_INVALID_ARGUMENT;
+    hipError_t e = hipFree(model->mem);
+    delete model;
+    return e == hipSuccess ? NNGP_OK : NNGP_ERR_HIP;
+}
+
+static void carve_model_ws(Carver& c, const nngp_model* M, int64_t nq, WsHeader** hdr, char** kmem,
+                           size_t* kb, int32_t** nn, VecLayout* L) {
+    *hdr = c.take<WsHeader>(1);
+    *kb = knn_search_bytes(nq, M->d, M->m, M->precision, false);
+    *kmem = c.take<char>(*kb);
+    *nn = c.take<int32_t>((size_t)nq * M->m + 1);
+    *L = carve_vec(c, nq, M->m, M->d, 0, false, M->precision);
+}
+
+size_t nngp_model_predict_workspace_bytes(const nngp_model* model, int64_t max_query) {
+    if (!model || max_query < 0) return 0;
+    Carver c(nullptr, 0);
+    WsHeader* hdr;
+    char* kmem;
+    size_t kb;
+    int32_t* nn;
+    VecLayout L;
+    carve_model_ws(c, model, max_query, &hdr, &kmem, &kb, &nn, &L);
+    return c.used + 256;
+}
+
+int nngp_model_predict(const nngp_model* model, const double* x_query, int64_t n_query,
+                       double* out_mean, double* out_std, void* workspace, size_t workspace_bytes,
+                       void* stream) {
+    hipStream_t st = (hipStream_t)stream;
+    if (!model || n_query < 0 || (n_query > 0 && (!x_query || !out_mean)) || !workspace)
+        return NNGP_ERR_INVALID_ARGUMENT;
+    const nngp_model* M = model;
+    Carver c(workspace, workspace_bytes);
+    WsHeader* hdr;
+    char* kmem;
+    size_t kb;
+    int32_t* nn;
+    VecLayout L;
+    carve_model_ws(c, M, n_query, &hdr, &kmem, &kb, &nn, &L);
+    if (!c.ok()) return NNGP_ERR_WORKSPACE_TOO_SMALL;
+    L.hdr = hdr;
+    hipLaunchKernelGGL(zero_header_kernel, dim3(1), dim3(64), 0, st, hdr);
+    if (n_query == 0) return NNGP_OK;
+    int rc = knn_search(M->refs, x_query, n_query, M->m, 0, 0, nn, kmem, kb, hdr, st);
+    if (rc) return rc;
+    VecArgs a{};
+    a.x = M->x;
+    a.y = M->y;
+    a.xs = x_query;
+    a.ys = nullptr;
+    a.nn = nn;
+    a.npts = n_query;
+    a.d = M->d;
+    a.ny = M->ny;
+    a.m = M->m;
+    a.alpha = M->alpha;
+    a.alpha_self = 0;
+    a.kp = M->kp;
+    a.n_theta = 0;
+    a.grad = 0;
+    a.mean = out_mean;
+    a.stdv = out_std;
+    a.hdr = hdr;
+    rc = run_fused(a, L, M->precision, st);
     if (rc) return rc;
     NNGP_HIP_CHECK(hipGetLastError());
     return NNGP_OK;

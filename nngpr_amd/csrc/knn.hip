@@ -152,6 +152,19 @@ __global__ void prep_kernel(PrepArgs a, const long long* mm, int64_t npad, int d
     }
 }
 
+// |u_i - centre|^2 in FP64 for points whose scaled coordinates are already in u64.
+__global__ void p2_kernel(const double* u64, int64_t n, int d, const long long* mm, double* p2) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double pp = 0.0;
+    for (int f = 0; f < d; ++f) {
+        double c = 0.5 * (key_dbl(mm[2 * f]) + key_dbl(mm[2 * f + 1]));
+        double p = u64[i * d + f] - c;
+        pp += p * p;
+    }
+    p2[i] = pp;
+}
+
 // ---------------------------------------------------------------------------
 // Streaming scan (exact FP64 or FP32 screen)
 // ---------------------------------------------------------------------------
@@ -681,30 +694,6 @@ inline int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 // Rows whose margin turns out too thin fail the certificate and are redone.
 inline int screen_K(int k) { return (int)round_up(k + 8, 8); }
 
-struct KnnLayout {
-    WsHeader* hdr;
-    long long* mm;
-    unsigned int* smax;
-    unsigned long long* flag_count;
-    double* ru64;
-    double* rs64;
-    float* rs32;
-    double* rp2;
-    double* qu64;
-    double* qs64;
-    float* qs32;
-    double* qp2;
-    int32_t* cand;
-    float* thr;
-    int64_t* flag_rows;
-    double* flag_key;
-    int32_t* flag_idx;
-    int64_t* over_rows;
-    unsigned long long* over_count;
-    double* lkey;
-    int32_t* lidx;
-};
-
 bool use_fast_scan(int d, int k) { return d >= 1 && d <= 8 && k >= 1 && k <= 64; }
 
 // Feature count the scan kernels are compiled for; extra features are zero in
@@ -713,49 +702,6 @@ inline int scan_dim(int d) { return d <= 3 ? 3 : (d == 4 ? 4 : 8); }
 
 // Exact-scan list length for k neighbours.
 inline int exact_K(int k) { return k <= 16 ? 16 : (k <= 32 ? 32 : 64); }
-
-KnnLayout carve_knn(void* ws, size_t bytes, int64_t n_ref, int64_t n_query, int d, int k,
-                    int precision, bool same, Carver& c) {
-    KnnLayout L{};
-    const int64_t rpad = round_up(std::max<int64_t>(n_ref, 1), 64) + 64;
-    const int64_t qpad = round_up(std::max<int64_t>(n_query, 1), 64) + 64;
-    const bool fast = use_fast_scan(d, k);
-    const bool screen = fast && precision == NNGP_PRECISION_FP32_REFINED;
-    L.hdr = c.take<WsHeader>(1);
-    L.mm = c.take<long long>(2 * NNGP_MAX_DIM);
-    L.smax = c.take<unsigned int>(4);
-    L.flag_count = c.take<unsigned long long>(4);
-    const int dp = scan_dim(d);
-    L.ru64 = c.take<double>((size_t)n_ref * d + 1);
-    if (fast && !screen) L.rs64 = c.take<double>((size_t)rpad * dp);
-    if (screen) {
-        L.rs32 = c.take<float>((size_t)rpad * (dp + 1));
-        L.rp2 = c.take<double>((size_t)n_ref + 1);
-    }
-    if (!same) {
-        L.qu64 = c.take<double>((size_t)n_query * d + 1);
-        if (fast && !screen) L.qs64 = c.take<double>((size_t)qpad * dp);
-        if (screen) {
-            L.qs32 = c.take<float>((size_t)qpad * (dp + 1));
-            L.qp2 = c.take<double>((size_t)n_query + 1);
-        }
-    }
-    if (screen) {
-        const int K = screen_K(k);
-        L.cand = c.take<int32_t>((size_t)n_query * K + 1);
-        L.thr = c.take<float>((size_t)n_query + 1);
-        L.flag_rows = c.take<int64_t>((size_t)n_query + 1);
-        L.flag_key = c.take<double>((size_t)n_query + 1);
-        L.flag_idx = c.take<int32_t>((size_t)n_query + 1);
-        L.over_rows = c.take<int64_t>((size_t)n_query + 1);
-        L.over_count = c.take<unsigned long long>(4);
-    }
-    L.lkey = c.take<double>((size_t)k * kFallbackThreads);
-    L.lidx = c.take<int32_t>((size_t)k * kFallbackThreads);
-    (void)ws;
-    (void)bytes;
-    return L;
-}
 
 template <typename T, int D, int K, bool SCREEN>
 void launch_scan_u(const ScanArgs<T>& a, hipStream_t st) {
@@ -820,99 +766,205 @@ int launch_rerank(const RerankArgs& a, int K, hipStream_t st) {
 
 }  // namespace
 
-size_t knn_workspace_bytes(int64_t n_ref, int64_t n_query, int d, int k, int precision,
-                           bool same) {
-    Carver c(nullptr, 0);
-    carve_knn(nullptr, 0, n_ref, n_query, d, k, precision, same, c);
-    return c.used + 256;
+// ---------------------------------------------------------------------------
+// Ref-side preparation (kept resident by serving models) and query-side search
+// ---------------------------------------------------------------------------
+namespace {
+
+struct RefMem {
+    long long* mm;
+    unsigned int* smax;
+    double* u64;
+    double* s64;
+    float* s32;
+};
+
+RefMem carve_refs(Carver& c, int64_t n_ref, int d, int precision) {
+    RefMem M{};
+    const int64_t rpad = round_up(std::max<int64_t>(n_ref, 1), 64) + 64;
+    const int dp = scan_dim(d);
+    M.mm = c.take<long long>(2 * NNGP_MAX_DIM);
+    M.smax = c.take<unsigned int>(4);
+    M.u64 = c.take<double>((size_t)n_ref * d + 1);
+    if (d <= 8) {
+        if (precision == NNGP_PRECISION_FP32_REFINED) M.s32 = c.take<float>((size_t)rpad * (dp + 1));
+        else M.s64 = c.take<double>((size_t)rpad * dp);
+    }
+    return M;
 }
 
-int knn_run(const double* ref, int64_t n_ref, const double* query, int64_t n_query, int d,
-            const double* scale, int n_scale, int k, int has_cut, int64_t ref_cut, int precision,
-            int32_t* out_idx, void* ws, size_t ws_bytes, hipStream_t st, bool reset_stats) {
-    if (d < 1 || d > NNGP_MAX_DIM || k < 1 || k > NNGP_MAX_NN || n_ref < 0 || n_query < 0)
-        return NNGP_ERR_INVALID_ARGUMENT;
+struct QueryMem {
+    double* u64;
+    double* s64;
+    float* s32;
+    double* p2;
+    int32_t* cand;
+    float* thr;
+    int64_t* flag_rows;
+    unsigned long long* flag_count;
+    double* flag_key;
+    int32_t* flag_idx;
+    int64_t* over_rows;
+    unsigned long long* over_count;
+    double* lkey;
+    int32_t* lidx;
+};
+
+QueryMem carve_query(Carver& c, int64_t n_query, int d, int k, int precision, bool same) {
+    QueryMem Q{};
+    const int64_t qpad = round_up(std::max<int64_t>(n_query, 1), 64) + 64;
+    const int dp = scan_dim(d);
+    const bool fast = use_fast_scan(d, k);
+    const bool screen = fast && precision == NNGP_PRECISION_FP32_REFINED;
+    if (!same) {
+        Q.u64 = c.take<double>((size_t)n_query * d + 1);
+        if (fast && !screen) Q.s64 = c.take<double>((size_t)qpad * dp);
+        if (screen) Q.s32 = c.take<float>((size_t)qpad * (dp + 1));
+    }
+    if (screen) {
+        Q.p2 = c.take<double>((size_t)n_query + 1);
+        const int K = screen_K(k);
+        Q.cand = c.take<int32_t>((size_t)n_query * K + 1);
+        Q.thr = c.take<float>((size_t)n_query + 1);
+        Q.flag_rows = c.take<int64_t>((size_t)n_query + 1);
+        Q.flag_key = c.take<double>((size_t)n_query + 1);
+        Q.flag_idx = c.take<int32_t>((size_t)n_query + 1);
+        Q.over_rows = c.take<int64_t>((size_t)n_query + 1);
+        Q.flag_count = c.take<unsigned long long>(4);
+        Q.over_count = c.take<unsigned long long>(4);
+    }
+    Q.lkey = c.take<double>((size_t)k * kFallbackThreads);
+    Q.lidx = c.take<int32_t>((size_t)k * kFallbackThreads);
+    return Q;
+}
+
+int check_common(int64_t n_ref, int d, int n_scale, int precision) {
+    if (d < 1 || d > NNGP_MAX_DIM || n_ref < 0) return NNGP_ERR_INVALID_ARGUMENT;
     if (n_scale != 0 && n_scale != 1 && n_scale != d) return NNGP_ERR_INVALID_ARGUMENT;
     if (n_ref >= (int64_t)INT32_MAX) return NNGP_ERR_UNSUPPORTED;
     if (precision != NNGP_PRECISION_FP64 && precision != NNGP_PRECISION_FP32_REFINED)
         return NNGP_ERR_INVALID_ARGUMENT;
-    const bool same = (ref == query && n_ref == n_query);
-    Carver c(ws, ws_bytes);
-    KnnLayout L = carve_knn(ws, ws_bytes, n_ref, n_query, d, k, precision, same, c);
-    if (!c.ok() || ws == nullptr) return NNGP_ERR_WORKSPACE_TOO_SMALL;
-    if (reset_stats) hipLaunchKernelGGL(zero_header_kernel, dim3(1), dim3(64), 0, st, L.hdr);
-    if (n_query == 0) return NNGP_OK;
+    return NNGP_OK;
+}
+
+}  // namespace
+
+size_t knn_refs_bytes(int64_t n_ref, int d, int precision) {
+    Carver c(nullptr, 0);
+    carve_refs(c, n_ref, d, precision);
+    return c.used + 256;
+}
+
+int knn_prepare_refs(const double* ref, int64_t n_ref, int d, const double* scale, int n_scale,
+                     int precision, void* mem, size_t bytes, KnnRefs* out, hipStream_t st) {
+    int rc = check_common(n_ref, d, n_scale, precision);
+    if (rc) return rc;
+    Carver c(mem, bytes);
+    RefMem M = carve_refs(c, n_ref, d, precision);
+    if (!c.ok() || !mem) return NNGP_ERR_WORKSPACE_TOO_SMALL;
+    KnnRefs R{};
+    R.u64 = M.u64;
+    R.s64 = M.s64;
+    R.s32 = M.s32;
+    R.mm = M.mm;
+    R.smax = M.smax;
+    R.n = n_ref;
+    R.d = d;
+    R.precision = precision;
+    R.n_scale = n_scale;
+    for (int f = 0; f < n_scale; ++f) R.scale[f] = scale[f];
+    *out = R;
+    hipLaunchKernelGGL(init_minmax_kernel, dim3(1), dim3(64), 0, st, M.mm, d, M.smax);
     if (n_ref == 0) {
+        NNGP_HIP_CHECK(hipGetLastError());
+        return NNGP_OK;
+    }
+    PrepArgs pr{};
+    pr.x = ref;
+    pr.n = n_ref;
+    pr.d = d;
+    pr.n_scale = n_scale;
+    for (int f = 0; f < n_scale; ++f) pr.scale[f] = scale[f];
+    if (M.s32) hipLaunchKernelGGL(minmax_kernel, dim3(256), dim3(256), 0, st, pr, M.mm);
+    const int64_t rpad = round_up(n_ref, 64) + 64;
+    hipLaunchKernelGGL(prep_kernel, dim3((unsigned)((rpad + 255) / 256)), dim3(256), 0, st, pr,
+                       M.mm, rpad, scan_dim(d), M.u64, M.s64, M.s32, (double*)nullptr, M.smax);
+    NNGP_HIP_CHECK(hipGetLastError());
+    return NNGP_OK;
+}
+
+size_t knn_search_bytes(int64_t n_query, int d, int k, int precision, bool same) {
+    Carver c(nullptr, 0);
+    carve_query(c, n_query, d, k, precision, same);
+    return c.used + 256;
+}
+
+int knn_search(const KnnRefs& R, const double* query, int64_t n_query, int k, int has_cut,
+               int64_t ref_cut, int32_t* out_idx, void* ws, size_t ws_bytes, WsHeader* hdr,
+               hipStream_t st) {
+    const int d = R.d;
+    if (k < 1 || k > NNGP_MAX_NN || n_query < 0) return NNGP_ERR_INVALID_ARGUMENT;
+    const bool same = query == nullptr;
+    Carver c(ws, ws_bytes);
+    QueryMem Q = carve_query(c, n_query, d, k, R.precision, same);
+    if (!c.ok() || (!ws && c.used > 0)) return NNGP_ERR_WORKSPACE_TOO_SMALL;
+    if (n_query == 0) return NNGP_OK;
+    if (R.n == 0) {
         const int64_t tot = n_query * (int64_t)k;
         hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
                            out_idx, tot, -1);
         NNGP_HIP_CHECK(hipGetLastError());
         return NNGP_OK;
     }
-    const bool fast = use_fast_scan(d, k);
-    const bool screen = fast && precision == NNGP_PRECISION_FP32_REFINED;
-    const int64_t rpad = round_up(n_ref, 64) + 64;
-    const int64_t qpad = round_up(n_query, 64) + 64;
-
-    PrepArgs pr{};
-    pr.d = d;
-    pr.n_scale = n_scale;
-    for (int f = 0; f < n_scale; ++f) pr.scale[f] = scale[f];
-
-    hipLaunchKernelGGL(init_minmax_kernel, dim3(1), dim3(64), 0, st, L.mm, d, L.smax);
-    if (screen) {
-        pr.x = ref;
-        pr.n = n_ref;
-        hipLaunchKernelGGL(minmax_kernel, dim3(256), dim3(256), 0, st, pr, L.mm);
-        if (!same) {
-            pr.x = query;
-            pr.n = n_query;
-            hipLaunchKernelGGL(minmax_kernel, dim3(256), dim3(256), 0, st, pr, L.mm);
-        }
-    }
-    pr.x = ref;
-    pr.n = n_ref;
+    const bool fast = use_fast_scan(d, k) && (R.s32 || R.s64);
+    const bool screen = fast && R.precision == NNGP_PRECISION_FP32_REFINED;
     const int dp = scan_dim(d);
-    hipLaunchKernelGGL(prep_kernel, dim3((unsigned)((rpad + 255) / 256)), dim3(256), 0, st, pr,
-                       L.mm, rpad, dp, L.ru64, L.rs64, L.rs32, L.rp2, L.smax);
-    const double* qu64 = L.ru64;
-    const double* qs64 = L.rs64;
-    const float* qs32 = L.rs32;
-    const double* qp2 = L.rp2;
+    const int64_t qpad = round_up(n_query, 64) + 64;
+    const double* qu64 = R.u64;
+    const double* qs64 = R.s64;
+    const float* qs32 = R.s32;
+    double* qp2 = Q.p2;
+    PrepArgs pr{};
+    pr.x = query;
+    pr.n = n_query;
+    pr.d = d;
+    pr.n_scale = R.n_scale;
+    for (int f = 0; f < R.n_scale; ++f) pr.scale[f] = R.scale[f];
     if (!same) {
-        pr.x = query;
-        pr.n = n_query;
         hipLaunchKernelGGL(prep_kernel, dim3((unsigned)((qpad + 255) / 256)), dim3(256), 0, st, pr,
-                           L.mm, qpad, dp, L.qu64, L.qs64, L.qs32, L.qp2, L.smax);
-        qu64 = L.qu64;
-        qs64 = L.qs64;
-        qs32 = L.qs32;
-        qp2 = L.qp2;
+                           R.mm, qpad, dp, Q.u64, Q.s64, Q.s32, Q.p2, (unsigned int*)nullptr);
+        qu64 = Q.u64;
+        qs64 = Q.s64;
+        qs32 = Q.s32;
+    } else if (screen) {
+        // |u_q - centre|^2 of the refs themselves (the query set is the ref set)
+        pr.x = nullptr;
+        hipLaunchKernelGGL(p2_kernel, dim3((unsigned)((n_query + 255) / 256)), dim3(256), 0, st,
+                           R.u64, n_query, d, R.mm, Q.p2);
     }
     NNGP_HIP_CHECK(hipGetLastError());
 
     if (!fast) {
         SimpleArgs sa{};
-        sa.ru64 = L.ru64;
+        sa.ru64 = R.u64;
         sa.qu64 = qu64;
         sa.d = d;
-        sa.n_ref = n_ref;
+        sa.n_ref = R.n;
         sa.n_query = n_query;
         sa.has_cut = has_cut;
         sa.cut = ref_cut;
         sa.k = k;
-        sa.lkey = L.lkey;
-        sa.lidx = L.lidx;
+        sa.lkey = Q.lkey;
+        sa.lidx = Q.lidx;
         sa.out_idx = out_idx;
         hipLaunchKernelGGL(simple_knn_kernel, dim3(kFallbackThreads / 256), dim3(256), 0, st, sa);
         NNGP_HIP_CHECK(hipGetLastError());
         return NNGP_OK;
     }
-
     if (!screen) {
         ScanArgs<double> a{};
-        a.ref = L.rs64;
-        a.n_ref = (int32_t)n_ref;
+        a.ref = R.s64;
+        a.n_ref = (int32_t)R.n;
         a.qry = qs64;
         a.n_query = n_query;
         a.has_cut = has_cut;
@@ -927,82 +979,112 @@ int knn_run(const double* ref, int64_t n_ref, const double* query, int64_t n_que
         NNGP_HIP_CHECK(hipGetLastError());
         return NNGP_OK;
     }
-
     const int K = screen_K(k);
     ScanArgs<float> a{};
-    a.ref = L.rs32;
-    a.n_ref = (int32_t)n_ref;
+    a.ref = R.s32;
+    a.n_ref = (int32_t)R.n;
     a.qry = qs32;
     a.n_query = n_query;
     a.has_cut = has_cut;
     a.cut = ref_cut;
     a.reverse = has_cut;
-    a.out_idx = L.cand;
-    a.out_thr = L.thr;
-    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, L.flag_count, 1);
+    a.out_idx = Q.cand;
+    a.out_thr = Q.thr;
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, Q.flag_count, 1);
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, Q.over_count, 1);
     timing_begin(TK_KNN_SCAN, st);
     const int rcs = launch_scan<float, true>(a, d, K, st);
     timing_end(TK_KNN_SCAN, st);
     if (rcs != 0) return NNGP_ERR_UNSUPPORTED;
 
     RerankArgs rr{};
-    rr.ru64 = L.ru64;
+    rr.ru64 = R.u64;
     rr.qu64 = qu64;
     rr.qp2 = qp2;
-    rr.smax_bits = L.smax;
+    rr.smax_bits = R.smax;
     rr.d = d;
     rr.n_query = n_query;
     rr.k = k;
-    rr.cand = L.cand;
-    rr.thr = L.thr;
+    rr.cand = Q.cand;
+    rr.thr = Q.thr;
     rr.out_idx = out_idx;
-    rr.flag_rows = L.flag_rows;
-    rr.flag_count = L.flag_count;
-    rr.flag_key = L.flag_key;
-    rr.flag_idx = L.flag_idx;
+    rr.flag_rows = Q.flag_rows;
+    rr.flag_count = Q.flag_count;
+    rr.flag_key = Q.flag_key;
+    rr.flag_idx = Q.flag_idx;
     rr.eps_factor = (4.0 * d + 16.0) * std::ldexp(1.0, -24);
     timing_begin(TK_KNN_RERANK, st);
     const int rcr = launch_rerank(rr, K, st);
     timing_end(TK_KNN_RERANK, st);
     if (rcr != 0) return NNGP_ERR_UNSUPPORTED;
 
-    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, L.over_count, 1);
     WaveFbArgs wf{};
-    wf.ru64 = L.ru64;
+    wf.ru64 = R.u64;
     wf.qu64 = qu64;
     wf.d = d;
-    wf.n_ref = n_ref;
+    wf.n_ref = R.n;
     wf.has_cut = has_cut;
     wf.cut = ref_cut;
     wf.k = k;
-    wf.rows = L.flag_rows;
-    wf.row_count = L.flag_count;
-    wf.bound_key = L.flag_key;
-    wf.bound_idx = L.flag_idx;
+    wf.rows = Q.flag_rows;
+    wf.row_count = Q.flag_count;
+    wf.bound_key = Q.flag_key;
+    wf.bound_idx = Q.flag_idx;
     wf.out_idx = out_idx;
-    wf.over_rows = L.over_rows;
-    wf.over_count = L.over_count;
+    wf.over_rows = Q.over_rows;
+    wf.over_count = Q.over_count;
     hipLaunchKernelGGL(wave_fallback_kernel, dim3(2048), dim3(64 * kFbWaves), 0, st, wf);
 
     SimpleArgs sa{};
-    sa.ru64 = L.ru64;
+    sa.ru64 = R.u64;
     sa.qu64 = qu64;
     sa.d = d;
-    sa.n_ref = n_ref;
+    sa.n_ref = R.n;
     sa.n_query = n_query;
     sa.has_cut = has_cut;
     sa.cut = ref_cut;
     sa.k = k;
-    sa.rows = L.over_rows;
-    sa.row_count = L.over_count;
-    sa.lkey = L.lkey;
-    sa.lidx = L.lidx;
+    sa.rows = Q.over_rows;
+    sa.row_count = Q.over_count;
+    sa.lkey = Q.lkey;
+    sa.lidx = Q.lidx;
     sa.out_idx = out_idx;
     hipLaunchKernelGGL(simple_knn_kernel, dim3(kFallbackThreads / 256), dim3(256), 0, st, sa);
-    hipLaunchKernelGGL(add_count_kernel, dim3(1), dim3(1), 0, st, L.flag_count,
-                       &L.hdr->knn_rows_recomputed);
+    if (hdr) hipLaunchKernelGGL(add_count_kernel, dim3(1), dim3(1), 0, st, Q.flag_count,
+                                &hdr->knn_rows_recomputed);
     NNGP_HIP_CHECK(hipGetLastError());
     return NNGP_OK;
+}
+
+size_t knn_workspace_bytes(int64_t n_ref, int64_t n_query, int d, int k, int precision,
+                           bool same) {
+    Carver c(nullptr, 0);
+    c.take<WsHeader>(1);
+    return c.used + knn_refs_bytes(n_ref, d, precision) +
+           knn_search_bytes(n_query, d, k, precision, same) + 512;
+}
+
+int knn_run(const double* ref, int64_t n_ref, const double* query, int64_t n_query, int d,
+            const double* scale, int n_scale, int k, int has_cut, int64_t ref_cut, int precision,
+            int32_t* out_idx, void* ws, size_t ws_bytes, hipStream_t st, bool reset_stats) {
+    int rc = check_common(n_ref, d, n_scale, precision);
+    if (rc) return rc;
+    if (k < 1 || k > NNGP_MAX_NN || n_query < 0) return NNGP_ERR_INVALID_ARGUMENT;
+    if (!ws) return NNGP_ERR_WORKSPACE_TOO_SMALL;
+    const bool same = (ref == query && n_ref == n_query);
+    Carver c(ws, ws_bytes);
+    WsHeader* hdr = c.take<WsHeader>(1);
+    const size_t rb = knn_refs_bytes(n_ref, d, precision);
+    char* rmem = c.take<char>(rb);
+    const size_t qb = knn_search_bytes(n_query, d, k, precision, same);
+    char* qmem = c.take<char>(qb);
+    if (!c.ok()) return NNGP_ERR_WORKSPACE_TOO_SMALL;
+    if (reset_stats) hipLaunchKernelGGL(zero_header_kernel, dim3(1), dim3(64), 0, st, hdr);
+    KnnRefs R;
+    rc = knn_prepare_refs(ref, n_ref, d, scale, n_scale, precision, rmem, rb, &R, st);
+    if (rc) return rc;
+    return knn_search(R, same ? nullptr : query, n_query, k, has_cut, ref_cut, out_idx, qmem, qb,
+                      hdr, st);
 }
 
 }  // namespace nngp
