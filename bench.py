@@ -166,32 +166,62 @@ def main():
     for _ in range(args.warmup):
         evaluate(F32)
     torch.cuda.synchronize()
-    nat.timing_enable(True)
-    for k in range(4):
-        nat.timing_read(k, reset=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        evaluate(F32)
-    torch.cuda.synchronize()
-    step_s = (time.perf_counter() - t0) / args.steps
-    nat.timing_enable(False)
-    scan_ms, scan_n = nat.timing_read(nat.TIMED_KNN_SCAN)
-    rerank_ms, _ = nat.timing_read(nat.TIMED_KNN_RERANK)
-    fused_ms, fused_n = nat.timing_read(nat.TIMED_LOGLIK)
-    scan_avg = scan_ms / max(scan_n, 1)
-    fused_avg = fused_ms / max(fused_n, 1)
+    TIMED = {"knn_prefix_scan": nat.TIMED_KNN_SCAN, "knn_rerank": nat.TIMED_KNN_RERANK,
+             "knn_grid_build": nat.TIMED_KNN_GRID_BUILD, "knn_grid_search": nat.TIMED_KNN_GRID,
+             "fused_loglik_grad": nat.TIMED_LOGLIK}
 
-    pairs = n * (n - 1) / 2.0
-    knn_flops = 3.0 * d * pairs  # sub, mul, add per feature per pair
-    achieved = knn_flops / (scan_avg * 1e-3) / 1e12
-    roofline = {"kernel": "knn screen scan (FP32 VALU, |r|^2 - 2 q.r)", "bound": "valu",
+    def stage_times(fn, steps):
+        """Per-step milliseconds of each timed kernel group over `steps` calls of fn
+        (HIP events on the launch stream), and the wall time per step."""
+        nat.timing_enable(True)
+        for k in TIMED.values():
+            nat.timing_read(k, reset=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / steps
+        nat.timing_enable(False)
+        out = {}
+        for name, k in TIMED.items():
+            ms, cnt = nat.timing_read(k)
+            out[name] = ms / steps
+            out[name + "_launches"] = cnt / steps
+        return wall, out
+
+    step_s, stages = stage_times(lambda: evaluate(F32), args.steps)
+    fused_avg = stages["fused_loglik_grad"] / max(stages["fused_loglik_grad_launches"], 1)
+
+    # Dominant kernel of the step: the fused fill -> Cholesky -> solve -> loglik ->
+    # adjoint pass.  Algorithmic flop per point (m neighbours, d features,
+    # E = (m+1)(m+2)/2 covariance entries): Cholesky 2m^3/3, two triangular solves
+    # 4m^2, fill E(3d+2), adjoint E(d+3) + 2m^2.
+    E = (m + 1) * (m + 2) / 2.0
+    fused_flop_pt = 2.0 * m ** 3 / 3.0 + 4.0 * m * m + E * (3 * d + 2) + E * (d + 3) + 2.0 * m * m
+    fused_flop = fused_flop_pt * n
+    achieved = fused_flop / (fused_avg * 1e-3) / 1e12
+    roofline = {"kernel": "vecchia_reg_kernel<float,32> fused loglik+grad (FP32 factor, FP64 fill "
+                          "and refinement)", "bound": "valu",
                 "achieved": achieved, "peak": PEAK_FP32_VALU_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / PEAK_FP32_VALU_TFLOPS, "traffic": None,
-                "avg_ms": scan_avg, "algorithmic_flop": knn_flops}
+                "avg_ms": fused_avg, "algorithmic_flop": fused_flop,
+                "algorithmic_flop_per_point": fused_flop_pt}
 
     extras = {}
     if not args.skip_extras:
+        # the same step with the brute-force kNN scans (FP32 screen + FP64 re-rank)
+        with hip_ops.knn_method(hip_ops.KNN_BRUTE_FORCE):
+            evaluate(F32)
+            tbf, stages_bf = stage_times(lambda: evaluate(F32), max(2, args.steps // 2))
+        scan_ms = stages_bf["knn_prefix_scan"]
+        pairs = n * (n - 1) / 2.0
+        knn_flops = 3.0 * d * pairs  # sub, mul, add per feature per pair
+        ach_scan = knn_flops / (scan_ms * 1e-3) / 1e12
+        roofline_scan = {"kernel": "knn brute-force screen scan (FP32 VALU, |r|^2 - 2 q.r)",
+                         "bound": "valu", "achieved": ach_scan, "peak": PEAK_FP32_VALU_TFLOPS,
+                         "unit": "TFLOP/s", "frac": ach_scan / PEAK_FP32_VALU_TFLOPS,
+                         "avg_ms": scan_ms, "algorithmic_flop": knn_flops}
         t64 = timed(lambda: evaluate(F64), max(2, args.steps // 2), 1)
         ll64, g64, nn64 = evaluate(F64)
         ll32, g32, nn32 = evaluate(F32)
@@ -232,9 +262,12 @@ def main():
             },
             "knn_indices_identical_fp32_vs_fp64": knn_exact,
             "knn_rows_recomputed_fp32": knn_stats["knn_rows_recomputed"] if knn_stats else None,
-            "stage_ms_fp32": {"knn_scan": scan_avg, "knn_rerank": rerank_ms / max(scan_n, 1),
-                              "fused_loglik_grad": fused_avg},
+            "stage_ms_fp32": {k: v for k, v in stages.items() if not k.endswith("_launches")},
             "stage_ms_fp64": {"fused_loglik_grad": f64_ms},
+            "evals_per_s_fp32_refined_brute_force_knn": 1.0 / tbf,
+            "stage_ms_fp32_brute_force_knn": {k: v for k, v in stages_bf.items()
+                                              if not k.endswith("_launches")},
+            "roofline_knn_brute_force_scan": roofline_scan,
             "loglik_fp64": ll64.item(),
         }
 

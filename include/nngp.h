@@ -86,13 +86,30 @@ int nngp_abi_version(void);
  * Key = FP64 squared distance of scaled coordinates, accumulated with FMA over
  * features in order; ties keep the lower index.  Both precisions return the
  * same indices: FP32_REFINED screens in FP32 and re-ranks a candidate margin in
- * FP64, certifying every row (uncertified rows are redone exactly).
+ * FP64, certifying every row (uncertified rows are redone exactly); the grid
+ * search (see nngp_knn_set_method) computes FP64 keys in both.
  */
 size_t nngp_knn_workspace_bytes(int64_t n_ref, int64_t n_query, int d, int k, int precision);
 int nngp_knn(const double* ref, int64_t n_ref, const double* query, int64_t n_query, int d,
              const double* scale, int n_scale, int k, int has_cut, int64_t ref_cut,
              int precision, int32_t* out_idx, void* workspace, size_t workspace_bytes,
              void* stream);
+
+/*
+ * Search method of nngp_knn and of the serving model's query search,
+ * process-wide, read when the refs are prepared (at each nngp_knn call, at
+ * nngp_model_create).  AUTO: a uniform-grid exact search for d <= 3 and k <= 64
+ * (the lowest Vecchia rows scanned brute force), the brute-force scans
+ * otherwise.  BRUTE_FORCE: always the scans.  Every method returns the same
+ * indices.  No reference counterpart (the reference has the brute-force scan
+ * only, cuda_kernel_accelerators.py:663-745).
+ */
+typedef enum {
+    NNGP_KNN_AUTO = 0,
+    NNGP_KNN_BRUTE_FORCE = 1
+} nngp_knn_method;
+int nngp_knn_set_method(int method);
+int nngp_knn_get_method(void);
 
 /*
  * Materialised neighbour covariance blocks (and theta gradients).
@@ -181,8 +198,9 @@ int nngp_read_stats(const void* workspace, nngp_stats* out, void* stream);
 
 /*
  * Kernel timing with HIP events recorded on the launch stream around the main
- * kernels: 0 = kNN scan, 1 = kNN re-rank, 2 = fused log-likelihood pass,
- * 3 = fused prediction pass.  Off by default; nngp_timing_read synchronises on
+ * kernels: 0 = kNN brute-force scan, 1 = kNN re-rank, 2 = fused log-likelihood
+ * pass, 3 = fused prediction pass, 4 = kNN grid search, 5 = kNN grid build (all
+ * its kernels).  Off by default; nngp_timing_read synchronises on
  * the recorded events and returns the summed milliseconds and launch count.
  * No reference counterpart (measurement support).
  */

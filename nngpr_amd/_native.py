@@ -72,11 +72,26 @@ SIGNATURES = [
     ("nngp_model_destroy", _I, [_P]),
     ("nngp_model_predict_workspace_bytes", _SZ, [_P, _I64]),
     ("nngp_model_predict", _I, [_P, _P, _I64, _P, _P, _P, _SZ, _P]),
+    ("nngp_knn_set_method", _I, [_I]),
+    ("nngp_knn_get_method", _I, []),
     ("nngp_timing_enable", _I, [_I]),
     ("nngp_timing_read", _I, [_I, _I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I64)]),
 ]
 
 TIMED_KNN_SCAN, TIMED_KNN_RERANK, TIMED_LOGLIK, TIMED_PREDICT = 0, 1, 2, 3
+TIMED_KNN_GRID, TIMED_KNN_GRID_BUILD = 4, 5
+
+KNN_AUTO, KNN_BRUTE_FORCE = 0, 1
+
+
+def knn_set_method(method):
+    """Process-wide kNN search method (KNN_AUTO: grid search for d <= 3, k <= 64;
+    KNN_BRUTE_FORCE: always the scans).  Both return identical indices."""
+    check(lib().nngp_knn_set_method(int(method)), "nngp_knn_set_method")
+
+
+def knn_get_method():
+    return int(lib().nngp_knn_get_method())
 
 
 def timing_enable(on=True):

@@ -268,6 +268,10 @@ int nngp_knn(const double* ref, int64_t n_ref, const double* query, int64_t n_qu
                    out_idx, workspace, workspace_bytes, (hipStream_t)stream, true);
 }
 
+int nngp_knn_set_method(int method) { return knn_set_method(method); }
+
+int nngp_knn_get_method(void) { return knn_method(); }
+
 int nngp_fill_nn_kernel(const double* x, int64_t n_x, int d, const int32_t* nn,
                         int64_t n_points, int width, const nngp_kernel* kernel, double* out_k,
                         double* out_grad, void* stream) {
@@ -457,7 +461,7 @@ int nngp_model_destroy(nngp_model* model) {
 static void carve_model_ws(Carver& c, const nngp_model* M, int64_t nq, WsHeader** hdr, char** kmem,
                            size_t* kb, int32_t** nn, VecLayout* L) {
     *hdr = c.take<WsHeader>(1);
-    *kb = knn_search_bytes(nq, M->d, M->m, M->precision, false);
+    *kb = knn_search_bytes(M->n, nq, M->d, M->m, M->precision, false, false);
     *kmem = c.take<char>(*kb);
     *nn = c.take<int32_t>((size_t)nq * M->m + 1);
     *L = carve_vec(c, nq, M->m, M->d, 0, false, M->precision);

@@ -24,7 +24,15 @@ namespace nngp {
 void set_last_hip_error(hipError_t e, const char* what, int line);
 
 // Optional per-kernel timing with HIP events on the launch stream (off by default).
-enum TimedKernel { TK_KNN_SCAN = 0, TK_KNN_RERANK = 1, TK_LOGLIK = 2, TK_PREDICT = 3, TK_COUNT = 4 };
+enum TimedKernel {
+    TK_KNN_SCAN = 0,
+    TK_KNN_RERANK = 1,
+    TK_LOGLIK = 2,
+    TK_PREDICT = 3,
+    TK_KNN_GRID = 4,
+    TK_KNN_GRID_BUILD = 5,
+    TK_COUNT = 6
+};
 void timing_begin(int id, hipStream_t st);
 void timing_end(int id, hipStream_t st);
 

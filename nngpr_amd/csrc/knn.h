@@ -1,17 +1,98 @@
-// Internal interface of the kNN module (knn.hip).
+// Internal interface of the kNN module (knn.hip, knn_grid.hip).
 #pragma once
 #include "common.h"
 
 namespace nngp {
 
-// Ref-side state of the kNN search: scaled coordinates (AoS), blocked scan
-// layout (FP32 centred + norms for the screen, or FP64), centre and max radius.
+// ---------------------------------------------------------------------------
+// Uniform-grid exact search (d <= 3, k <= 64), knn_grid.hip
+// ---------------------------------------------------------------------------
+// One point of a cell-sorted array: scaled coordinates (unused features 0) and
+// the point's row index.
+struct alignas(32) GridPt {
+    double u[3];
+    long long idx;
+};
+
+// Grid geometry, computed on the device from the bounding box of the finite refs.
+// Cell of a point: c_f = clamp(floor((u_f - lo_f) * inv_h), 0, g_f - 1); linear id
+// c_0 + g_0 * (c_1 + g_1 * c_2) (refs are stored in this order).  Features >= d
+// have g_f = 1.  Queries are ordered by tiles of t_0 x t_1 x t_2 cells (tg_f tiles
+// per feature, n_qb buckets), so that consecutive queries are spatially compact.
+struct GridParams {
+    double lo[3];
+    double h;
+    double inv_h;
+    double tol;       // slack subtracted from every distance lower bound
+    int g[3];
+    int n_cells;
+    int t[3];
+    int tg[3];
+    int n_qb;
+};
+
+// A grid over a set of refs and the refs sorted by cell.  Bucket `cap` collects
+// points with a non-finite coordinate (never neighbours); start has cap + 2
+// entries.
+struct GridMem {
+    GridParams* params;
+    int32_t* cnt;
+    int32_t* start;
+    int32_t* fill;
+    int32_t* sums;
+    int32_t* cell_of;
+    GridPt* pts;
+    int cap;
+};
+
+// Ref side, kept with the refs: bounding box and the grid over all refs.
+struct GridRefMem {
+    long long* bbox;
+    GridMem full;
+};
+
+// Per search: a grid over a prefix of the refs (lower Vecchia bands) and the
+// queries of one band sorted by tile.
+struct GridSearchMem {
+    GridMem band;
+    int32_t* qcnt;
+    int32_t* qstart;
+    int32_t* qfill;
+    int32_t* qsums;
+    int32_t* qcell_of;
+    GridPt* qpts;
+    int32_t* qitem;   // tile of each work item (64 queries of one tile)
+    int capq;
+};
+
+bool grid_supported(int d, int k);
+// Vecchia rows whose ref limit is below this are scanned brute force.
+int64_t grid_prefix_limit(int64_t n_ref, int k);
+void carve_grid_refs(Carver& c, int64_t n_ref, GridRefMem* M);
+void carve_grid_search(Carver& c, int64_t n_ref, int64_t n_query, bool with_cut,
+                       GridSearchMem* M);
+// Builds the grid over n scaled points (u64, n x d AoS).
+int grid_build(const double* u64, int64_t n, int d, const GridRefMem& M, hipStream_t st);
+// Exact top-k of query rows [row0, n_query) (rows below have limits under
+// grid_prefix_limit and are scanned elsewhere; row0 = 0 without a cut).
+int grid_search(const GridRefMem& G, const double* ru64, int64_t n_ref, int d,
+                const double* qu64, int64_t n_query, int k, int has_cut, int64_t cut,
+                int64_t row0, const GridSearchMem& M, int32_t* out_idx, hipStream_t st);
+
+// ---------------------------------------------------------------------------
+// Ref-side state of the kNN search
+// ---------------------------------------------------------------------------
+// Scaled coordinates (AoS), and either the uniform grid (d <= 3 by default) or
+// the blocked scan layout (FP32 centred + norms for the screen, or FP64), with
+// the centre and max radius.
 struct KnnRefs {
     const double* u64;
     const double* s64;
     const float* s32;
     const long long* mm;
     const unsigned int* smax;
+    GridRefMem grid;
+    int use_grid;
     int64_t n;
     int d;
     int precision;
@@ -19,10 +100,18 @@ struct KnnRefs {
     double scale[NNGP_MAX_DIM];
 };
 
+// Process-wide search method: NNGP_KNN_AUTO or NNGP_KNN_BRUTE_FORCE.  Read when
+// refs are prepared.
+int knn_method();
+int knn_set_method(int method);
+
 size_t knn_refs_bytes(int64_t n_ref, int d, int precision);
 int knn_prepare_refs(const double* ref, int64_t n_ref, int d, const double* scale, int n_scale,
                      int precision, void* mem, size_t bytes, KnnRefs* out, hipStream_t st);
-size_t knn_search_bytes(int64_t n_query, int d, int k, int precision, bool same);
+// with_cut: budget for searches with a Vecchia cut (nngp_knn); the serving
+// model's query searches have none.
+size_t knn_search_bytes(int64_t n_ref, int64_t n_query, int d, int k, int precision, bool same,
+                        bool with_cut);
 // query == NULL searches the ref set itself (Vecchia self-search).
 int knn_search(const KnnRefs& R, const double* query, int64_t n_query, int k, int has_cut,
                int64_t ref_cut, int32_t* out_idx, void* ws, size_t ws_bytes, WsHeader* hdr,

@@ -18,11 +18,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 
 #include "common.h"
 #include "knn.h"
+#include "knn_device.h"
 
 namespace nngp {
 
@@ -38,16 +40,6 @@ struct PrepArgs {
     double scale[NNGP_MAX_DIM];
 };
 
-// Monotone map of a double onto int64 so that atomicMin/atomicMax order like the
-// doubles (NaN excluded).
-__device__ __forceinline__ long long dbl_key(double v) {
-    long long b = __double_as_longlong(v);
-    return b >= 0 ? b : (b ^ 0x7fffffffffffffffLL);
-}
-__device__ __forceinline__ double key_dbl(long long k) {
-    long long b = k >= 0 ? k : (k ^ 0x7fffffffffffffffLL);
-    return __longlong_as_double(b);
-}
 
 __device__ __forceinline__ double scaled(const PrepArgs& a, int64_t i, int f) {
     double v = a.x[i * a.d + f];
@@ -73,7 +65,7 @@ __global__ void minmax_kernel(PrepArgs a, long long* mm) {
         for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n;
              i += (int64_t)gridDim.x * blockDim.x) {
             double v = scaled(a, i, f);
-            if (v == v) {
+            if (__builtin_isfinite(v)) {  // the centre ignores NaN and infinite coordinates
                 long long k = dbl_key(v);
                 lo = k < lo ? k : lo;
                 hi = k > hi ? k : hi;
@@ -146,8 +138,10 @@ __global__ void prep_kernel(PrepArgs a, const long long* mm, int64_t npad, int d
         s32[blk8(i, dp, dp + 1)] = -(float)nrm;
         if (p2) p2[i] = pp;
         if (smax) {
+            // a point with a non-finite coordinate is never admitted by the
+            // screen, so it does not enter the radius bound
             float r = (float)sqrt(fmax(nrm, pp)) * 1.0001f;
-            atomicMax(smax, __float_as_uint(r));
+            if (__builtin_isfinite(r)) atomicMax(smax, __float_as_uint(r));
         }
     }
 }
@@ -165,6 +159,20 @@ __global__ void p2_kernel(const double* u64, int64_t n, int d, const long long* 
     p2[i] = pp;
 }
 
+// Blocked FP64 layout (nv = dp) of n scaled points given as AoS (n x d); entries
+// up to npad are padding (+inf in feature 0, never admitted).
+__global__ void blocked_from_aos_kernel(const double* u64, int64_t n, int64_t npad, int d, int dp,
+                                        double* s64) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npad) return;
+    for (int f = 0; f < dp; ++f) {
+        double v = 0.0;
+        if (i >= n) v = f == 0 ? __builtin_inf() : 0.0;
+        else if (f < d) v = u64[i * d + f];
+        s64[blk8(i, f, dp)] = v;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Streaming scan (exact FP64 or FP32 screen)
 // ---------------------------------------------------------------------------
@@ -177,39 +185,12 @@ struct ScanArgs {
     int has_cut;
     int64_t cut;
     int reverse;         // process the longest rows first
+    int64_t row_off;     // exact: row index of query 0 (its limit and output row)
     int k_out;           // exact: neighbours written per row (<= K)
     int32_t* out_idx;    // exact: n_query x k_out ; screen: n_query x K candidates
     float* out_thr;      // screen: K-th candidate score per row
 };
 
-__device__ __forceinline__ int32_t wave_min32(int32_t v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
-    return __builtin_amdgcn_readfirstlane(v);
-}
-__device__ __forceinline__ int32_t wave_max32(int32_t v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
-    return __builtin_amdgcn_readfirstlane(v);
-}
-
-template <typename T, int K>
-__device__ __forceinline__ void topk_insert(T (&keys)[K], int (&ids)[K], T xk, int xi) {
-    // Compare-exchange ladder over a sorted register list, ordered by (key, index):
-    // the carried element must move ahead of an equal key with a higher index.
-#pragma unroll
-    for (int t = 0; t < K; ++t) {
-        bool lt = xk < keys[t] || (xk == keys[t] && (unsigned)xi < (unsigned)ids[t]);
-        T nk = lt ? xk : keys[t];
-        T ok = lt ? keys[t] : xk;
-        int ni = lt ? xi : ids[t];
-        int oi = lt ? ids[t] : xi;
-        keys[t] = nk;
-        ids[t] = ni;
-        xk = ok;
-        xi = oi;
-    }
-}
 
 // Order-preserving map of an FP32 score onto uint32, packed with the ref index:
 // comparing the packed uint64 values orders by (score, index) and puts empty
@@ -271,7 +252,7 @@ __global__ __launch_bounds__(64) void scan_kernel(ScanArgs<T> a) {
     const bool qv = q < a.n_query;
     int64_t lim64 = a.n_ref;
     if (a.has_cut) {
-        lim64 = q + a.cut;
+        lim64 = q + a.row_off + a.cut;
         lim64 = lim64 < 0 ? 0 : (lim64 > a.n_ref ? a.n_ref : lim64);
     }
     const int32_t lim = qv ? (int32_t)lim64 : 0;
@@ -441,7 +422,7 @@ __global__ __launch_bounds__(64) void scan_kernel(ScanArgs<T> a) {
     } else {
 #pragma unroll
         for (int t = 0; t < KE; ++t)
-            if (t < a.k_out) a.out_idx[q * a.k_out + t] = ids[t];
+            if (t < a.k_out) a.out_idx[(q + a.row_off) * a.k_out + t] = ids[t];
     }
 }
 
@@ -466,15 +447,6 @@ struct RerankArgs {
     double eps_factor;    // (4 d + 16) * 2^-24
 };
 
-__device__ __forceinline__ double exact_key(const double* ur, const double* uq, int d) {
-    double t0 = ur[0] - uq[0];
-    double key = t0 * t0;
-    for (int f = 1; f < d; ++f) {
-        double t = ur[f] - uq[f];
-        key = fma(t, t, key);
-    }
-    return key;
-}
 
 template <int K>
 __global__ __launch_bounds__(256) void rerank_kernel(RerankArgs a) {
@@ -482,20 +454,23 @@ __global__ __launch_bounds__(256) void rerank_kernel(RerankArgs a) {
     if (q >= a.n_query) return;
     double key[K];
     int ids[K];
-    int count = 0;
+    int count = 0;   // candidates the screen kept
+    int nfin = 0;    // of which with a finite exact key (the others are never neighbours)
     const double* uq = a.qu64 + q * a.d;
 #pragma unroll
     for (int t = 0; t < K; ++t) {
         int id = a.cand[q * K + t];
-        ids[t] = id;
+        key[t] = __builtin_inf();
         if (id >= 0) {
             key[t] = exact_key(a.ru64 + (int64_t)id * a.d, uq, a.d);
             ++count;
-        } else {
-            key[t] = __builtin_inf();
+            if (!(key[t] < __builtin_inf())) id = -1;
         }
+        ids[t] = id;
+        nfin += id >= 0 ? 1 : 0;
     }
-    double dm = -__builtin_inf();
+    // m-th (key, index) pair; +inf when fewer than m finite keys (never certified)
+    double dm = __builtin_inf();
     int dmi = -1;
 #pragma unroll
     for (int t = 0; t < K; ++t) {
@@ -513,7 +488,7 @@ __global__ __launch_bounds__(256) void rerank_kernel(RerankArgs a) {
             dmi = ids[t];
         }
     }
-    for (int j = count; j < a.k; ++j) a.out_idx[q * a.k + j] = -1;
+    for (int j = nfin; j < a.k; ++j) a.out_idx[q * a.k + j] = -1;
     if (count < K) return;  // every ref of the row was a candidate: exact
     // Certificate: every non-candidate r had FP32 score >= thr, so its true key is
     // >= thr + |p|^2 - E.  Strictly above the m-th candidate key => exact top-m.
@@ -688,7 +663,6 @@ __global__ void fill_i32_kernel(int32_t* p, int64_t n, int32_t v) {
 // ---------------------------------------------------------------------------
 // Host dispatch
 // ---------------------------------------------------------------------------
-inline int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
 // Screen candidate count for k neighbours: k plus a margin of 8, rounded to 8.
 // Rows whose margin turns out too thin fail the certificate and are redone.
@@ -777,8 +751,11 @@ struct RefMem {
     double* u64;
     double* s64;
     float* s32;
+    GridRefMem grid;
 };
 
+// Both the scan layout and (d <= 3) the grid are budgeted, so that one size
+// serves either search method.
 RefMem carve_refs(Carver& c, int64_t n_ref, int d, int precision) {
     RefMem M{};
     const int64_t rpad = round_up(std::max<int64_t>(n_ref, 1), 64) + 64;
@@ -790,6 +767,7 @@ RefMem carve_refs(Carver& c, int64_t n_ref, int d, int precision) {
         if (precision == NNGP_PRECISION_FP32_REFINED) M.s32 = c.take<float>((size_t)rpad * (dp + 1));
         else M.s64 = c.take<double>((size_t)rpad * dp);
     }
+    if (d <= 3) carve_grid_refs(c, n_ref, &M.grid);
     return M;
 }
 
@@ -808,9 +786,13 @@ struct QueryMem {
     unsigned long long* over_count;
     double* lkey;
     int32_t* lidx;
+    GridSearchMem grid;
+    double* pre_r;   // grid search with a cut: blocked FP64 refs / queries of the
+    double* pre_q;   //   rows scanned brute force
 };
 
-QueryMem carve_query(Carver& c, int64_t n_query, int d, int k, int precision, bool same) {
+QueryMem carve_query(Carver& c, int64_t n_ref, int64_t n_query, int d, int k, int precision,
+                     bool same, bool with_cut) {
     QueryMem Q{};
     const int64_t qpad = round_up(std::max<int64_t>(n_query, 1), 64) + 64;
     const int dp = scan_dim(d);
@@ -835,8 +817,18 @@ QueryMem carve_query(Carver& c, int64_t n_query, int d, int k, int precision, bo
     }
     Q.lkey = c.take<double>((size_t)k * kFallbackThreads);
     Q.lidx = c.take<int32_t>((size_t)k * kFallbackThreads);
+    if (grid_supported(d, k)) {
+        carve_grid_search(c, n_ref, n_query, with_cut, &Q.grid);
+        if (with_cut) {
+            const int64_t pre = round_up(grid_prefix_limit(n_ref, k), 64) + 64;
+            Q.pre_r = c.take<double>((size_t)pre * scan_dim(d));
+            Q.pre_q = c.take<double>((size_t)pre * scan_dim(d));
+        }
+    }
     return Q;
 }
+
+std::atomic<int> g_knn_method{NNGP_KNN_AUTO};
 
 int check_common(int64_t n_ref, int d, int n_scale, int precision) {
     if (d < 1 || d > NNGP_MAX_DIM || n_ref < 0) return NNGP_ERR_INVALID_ARGUMENT;
@@ -848,6 +840,14 @@ int check_common(int64_t n_ref, int d, int n_scale, int precision) {
 }
 
 }  // namespace
+
+int knn_method() { return g_knn_method.load(); }
+
+int knn_set_method(int method) {
+    if (method != NNGP_KNN_AUTO && method != NNGP_KNN_BRUTE_FORCE) return NNGP_ERR_INVALID_ARGUMENT;
+    g_knn_method.store(method);
+    return NNGP_OK;
+}
 
 size_t knn_refs_bytes(int64_t n_ref, int d, int precision) {
     Carver c(nullptr, 0);
@@ -862,10 +862,17 @@ int knn_prepare_refs(const double* ref, int64_t n_ref, int d, const double* scal
     Carver c(mem, bytes);
     RefMem M = carve_refs(c, n_ref, d, precision);
     if (!c.ok() || !mem) return NNGP_ERR_WORKSPACE_TOO_SMALL;
+    const bool grid = d <= 3 && knn_method() == NNGP_KNN_AUTO;
+    if (grid) {  // the grid search needs the scaled coordinates only
+        M.s64 = nullptr;
+        M.s32 = nullptr;
+    }
     KnnRefs R{};
     R.u64 = M.u64;
     R.s64 = M.s64;
     R.s32 = M.s32;
+    R.grid = M.grid;
+    R.use_grid = grid ? 1 : 0;
     R.mm = M.mm;
     R.smax = M.smax;
     R.n = n_ref;
@@ -890,12 +897,14 @@ int knn_prepare_refs(const double* ref, int64_t n_ref, int d, const double* scal
     hipLaunchKernelGGL(prep_kernel, dim3((unsigned)((rpad + 255) / 256)), dim3(256), 0, st, pr,
                        M.mm, rpad, scan_dim(d), M.u64, M.s64, M.s32, (double*)nullptr, M.smax);
     NNGP_HIP_CHECK(hipGetLastError());
+    if (grid) return grid_build(M.u64, n_ref, d, M.grid, st);
     return NNGP_OK;
 }
 
-size_t knn_search_bytes(int64_t n_query, int d, int k, int precision, bool same) {
+size_t knn_search_bytes(int64_t n_ref, int64_t n_query, int d, int k, int precision, bool same,
+                        bool with_cut) {
     Carver c(nullptr, 0);
-    carve_query(c, n_query, d, k, precision, same);
+    carve_query(c, n_ref, n_query, d, k, precision, same, with_cut);
     return c.used + 256;
 }
 
@@ -906,7 +915,7 @@ int knn_search(const KnnRefs& R, const double* query, int64_t n_query, int k, in
     if (k < 1 || k > NNGP_MAX_NN || n_query < 0) return NNGP_ERR_INVALID_ARGUMENT;
     const bool same = query == nullptr;
     Carver c(ws, ws_bytes);
-    QueryMem Q = carve_query(c, n_query, d, k, R.precision, same);
+    QueryMem Q = carve_query(c, R.n, n_query, d, k, R.precision, same, has_cut != 0);
     if (!c.ok() || (!ws && c.used > 0)) return NNGP_ERR_WORKSPACE_TOO_SMALL;
     if (n_query == 0) return NNGP_OK;
     if (R.n == 0) {
@@ -931,11 +940,15 @@ int knn_search(const KnnRefs& R, const double* query, int64_t n_query, int k, in
     pr.n_scale = R.n_scale;
     for (int f = 0; f < R.n_scale; ++f) pr.scale[f] = R.scale[f];
     if (!same) {
+        // only the layouts the chosen search reads are written
+        double* qs64w = fast && !screen ? Q.s64 : nullptr;
+        float* qs32w = screen ? Q.s32 : nullptr;
         hipLaunchKernelGGL(prep_kernel, dim3((unsigned)((qpad + 255) / 256)), dim3(256), 0, st, pr,
-                           R.mm, qpad, dp, Q.u64, Q.s64, Q.s32, Q.p2, (unsigned int*)nullptr);
+                           R.mm, qpad, dp, Q.u64, qs64w, qs32w, screen ? Q.p2 : nullptr,
+                           (unsigned int*)nullptr);
         qu64 = Q.u64;
-        qs64 = Q.s64;
-        qs32 = Q.s32;
+        qs64 = qs64w;
+        qs32 = qs32w;
     } else if (screen) {
         // |u_q - centre|^2 of the refs themselves (the query set is the ref set)
         pr.x = nullptr;
@@ -944,6 +957,49 @@ int knn_search(const KnnRefs& R, const double* query, int64_t n_query, int k, in
     }
     NNGP_HIP_CHECK(hipGetLastError());
 
+    if (R.use_grid && grid_supported(d, k)) {
+        int64_t row0 = 0;
+        if (has_cut) {
+            // Rows whose limit q + cut is below the first grid band: brute-force
+            // exact scan over their few refs; rows with no ref at all get -1.
+            const int64_t L0 = grid_prefix_limit(R.n, k);
+            const int64_t qlo = std::min<int64_t>(std::max<int64_t>(-ref_cut, 0), n_query);
+            row0 = std::min<int64_t>(std::max<int64_t>(L0 - ref_cut, 0), n_query);
+            if (qlo > 0) {
+                const int64_t tot = qlo * (int64_t)k;
+                hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256),
+                                   0, st, out_idx, tot, -1);
+            }
+            if (row0 > qlo) {
+                const int64_t nr = std::min<int64_t>(R.n, L0);
+                const int64_t rpad = round_up(nr, 64) + 64;
+                const int64_t qpad = round_up(row0 - qlo, 64) + 64;
+                hipLaunchKernelGGL(blocked_from_aos_kernel, dim3((unsigned)((rpad + 255) / 256)),
+                                   dim3(256), 0, st, R.u64, nr, rpad, d, dp, Q.pre_r);
+                hipLaunchKernelGGL(blocked_from_aos_kernel, dim3((unsigned)((qpad + 255) / 256)),
+                                   dim3(256), 0, st, qu64 + qlo * d, row0 - qlo, qpad, d, dp,
+                                   Q.pre_q);
+                ScanArgs<double> a{};
+                a.ref = Q.pre_r;
+                a.n_ref = (int32_t)nr;
+                a.qry = Q.pre_q;
+                a.n_query = row0 - qlo;
+                a.has_cut = 1;
+                a.cut = ref_cut;
+                a.reverse = 1;
+                a.row_off = qlo;
+                a.k_out = k;
+                a.out_idx = out_idx;
+                timing_begin(TK_KNN_SCAN, st);
+                const int rc = launch_scan<double, false>(a, d, exact_K(k), st);
+                timing_end(TK_KNN_SCAN, st);
+                if (rc != 0) return NNGP_ERR_UNSUPPORTED;
+            }
+            NNGP_HIP_CHECK(hipGetLastError());
+        }
+        return grid_search(R.grid, R.u64, R.n, d, qu64, n_query, k, has_cut, ref_cut, row0,
+                           Q.grid, out_idx, st);
+    }
     if (!fast) {
         SimpleArgs sa{};
         sa.ru64 = R.u64;
@@ -1061,7 +1117,7 @@ size_t knn_workspace_bytes(int64_t n_ref, int64_t n_query, int d, int k, int pre
     Carver c(nullptr, 0);
     c.take<WsHeader>(1);
     return c.used + knn_refs_bytes(n_ref, d, precision) +
-           knn_search_bytes(n_query, d, k, precision, same) + 512;
+           knn_search_bytes(n_ref, n_query, d, k, precision, same, true) + 512;
 }
 
 int knn_run(const double* ref, int64_t n_ref, const double* query, int64_t n_query, int d,
@@ -1076,7 +1132,7 @@ int knn_run(const double* ref, int64_t n_ref, const double* query, int64_t n_que
     WsHeader* hdr = c.take<WsHeader>(1);
     const size_t rb = knn_refs_bytes(n_ref, d, precision);
     char* rmem = c.take<char>(rb);
-    const size_t qb = knn_search_bytes(n_query, d, k, precision, same);
+    const size_t qb = knn_search_bytes(n_ref, n_query, d, k, precision, same, has_cut != 0);
     char* qmem = c.take<char>(qb);
     if (!c.ok()) return NNGP_ERR_WORKSPACE_TOO_SMALL;
     if (reset_stats) hipLaunchKernelGGL(zero_header_kernel, dim3(1), dim3(64), 0, st, hdr);

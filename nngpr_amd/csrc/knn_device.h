@@ -1,0 +1,65 @@
+// Device helpers shared by the kNN kernels (brute-force scans and grid search).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nngp {
+
+inline int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+// Monotone map of a double onto int64 so that atomicMin/atomicMax order like the
+// doubles (NaN excluded).
+__device__ __forceinline__ long long dbl_key(double v) {
+    long long b = __double_as_longlong(v);
+    return b >= 0 ? b : (b ^ 0x7fffffffffffffffLL);
+}
+__device__ __forceinline__ double key_dbl(long long k) {
+    long long b = k >= 0 ? k : (k ^ 0x7fffffffffffffffLL);
+    return __longlong_as_double(b);
+}
+
+__device__ __forceinline__ int32_t wave_min32(int32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+__device__ __forceinline__ int32_t wave_max32(int32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+// Compare-exchange ladder over a sorted register list, ordered by (key, index):
+// the carried element moves ahead of an equal key with a higher index.  Indices
+// compare unsigned, so an empty slot (index -1) sorts after every real entry of
+// the same key.
+template <typename T, int K>
+__device__ __forceinline__ void topk_insert(T (&keys)[K], int (&ids)[K], T xk, int xi) {
+#pragma unroll
+    for (int t = 0; t < K; ++t) {
+        // bitwise, not short-circuit: keeps the ladder straight-line code
+        const bool lt = (xk < keys[t]) | ((xk == keys[t]) & ((unsigned)xi < (unsigned)ids[t]));
+        T nk = lt ? xk : keys[t];
+        T ok = lt ? keys[t] : xk;
+        int ni = lt ? xi : ids[t];
+        int oi = lt ? ids[t] : xi;
+        keys[t] = nk;
+        ids[t] = ni;
+        xk = ok;
+        xi = oi;
+    }
+}
+
+// FP64 key of a (ref, query) pair: fma(t_{d-1}, t_{d-1}, ... fma(t_1, t_1, t_0 * t_0)),
+// t_f = u_ref,f - u_query,f.
+__device__ __forceinline__ double exact_key(const double* ur, const double* uq, int d) {
+    double t0 = ur[0] - uq[0];
+    double key = t0 * t0;
+    for (int f = 1; f < d; ++f) {
+        double t = ur[f] - uq[f];
+        key = fma(t, t, key);
+    }
+    return key;
+}
+
+}  // namespace nngp

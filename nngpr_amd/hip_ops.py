@@ -13,6 +13,8 @@ from . import _native as nat
 
 PRECISION_FP64 = nat.PRECISION_FP64
 PRECISION_FP32_REFINED = nat.PRECISION_FP32_REFINED
+KNN_AUTO = nat.KNN_AUTO
+KNN_BRUTE_FORCE = nat.KNN_BRUTE_FORCE
 
 _workspaces = {}
 
@@ -206,3 +208,21 @@ def predict(x_train, y_train, x_query, nn, kstruct, alpha, precision=PRECISION_F
     if check_pd:
         _check_pd(ws, dev, "predict")
     return mean, std
+
+
+class knn_method:
+    """Context manager selecting the process-wide kNN search method:
+    `with hip_ops.knn_method(hip_ops.KNN_BRUTE_FORCE): ...`."""
+
+    def __init__(self, method):
+        self.method = int(method)
+        self.prev = None
+
+    def __enter__(self):
+        self.prev = nat.knn_get_method()
+        nat.knn_set_method(self.method)
+        return self
+
+    def __exit__(self, *exc):
+        nat.knn_set_method(self.prev)
+        return False

@@ -1,10 +1,11 @@
 """HIP kNN vs the FP64 oracle: neighbour indices must be bit-identical.
 
-Covers both search forms (exact FP64 scan; FP32 screen + FP64 re-rank with
-certificate), Vecchia cuts like the reference's tests (None, 0, 2, -1),
-isotropic / ARD / raw coordinates, ties and duplicates, empty inputs, k > 64 and
-d > 8 (per-row path), the stored golden vectors, and a slice of the 434,874-point
-road-mixture workload.
+Covers every search form (uniform-grid search for d <= 3; brute-force exact FP64
+scan; FP32 screen + FP64 re-rank with certificate), Vecchia cuts like the
+reference's tests (None, 0, 2, -1), isotropic / ARD / raw coordinates, ties and
+duplicates, non-finite coordinates, clustered and degenerate point sets, queries
+far outside the refs, empty inputs, k > 64 and d > 8 (per-row path), the stored
+golden vectors, and slices of the 434,874-point road-mixture workload.
 """
 import os
 
@@ -24,13 +25,28 @@ def t(a):
     return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(DEV)
 
 
+METHODS = [hip_ops.KNN_AUTO, hip_ops.KNN_BRUTE_FORCE]
+
+
+@pytest.fixture(params=METHODS, ids=["auto", "brute"])
+def method(request):
+    with hip_ops.knn_method(request.param):
+        yield request.param
+
+
 def hip_knn(ref, qry, ls, k, cut, prec):
     out = hip_ops.knn(t(ref), t(qry), ls, k, ref_cut=cut, precision=prec)
     return out.cpu().numpy()
 
 
+def hip_knn_self(x, ls, k, cut, prec):
+    """The query set is the ref tensor itself (the Vecchia self-search)."""
+    xt = t(x)
+    return hip_ops.knn(xt, xt, ls, k, ref_cut=cut, precision=prec).cpu().numpy()
+
+
 @pytest.mark.parametrize("prec", [0, 1])
-def test_golden_small_uniform(prec):
+def test_golden_small_uniform(prec, method):
     g = np.load(GOLD)
     ds = datasets.small_uniform()
     x, xq = ds["x_train"], ds["x_test"]
@@ -42,7 +58,7 @@ def test_golden_small_uniform(prec):
 @pytest.mark.parametrize("cut", [None, 0, 2, -1])
 @pytest.mark.parametrize("d,ls", [(1, 0.2), (2, None), (3, "ard"), (5, 0.7), (8, "ard")])
 @pytest.mark.parametrize("k", [1, 7, 33, 64])
-def test_against_oracle(prec, cut, d, ls, k):
+def test_against_oracle(prec, cut, d, ls, k, method):
     rng = np.random.default_rng(100 + d * 7 + k)
     ref = rng.normal(size=(2100, d))
     qry = ref[:700] if cut is not None else rng.normal(size=(700, d))
@@ -63,19 +79,20 @@ def test_per_row_path(prec, d, k):
 
 
 @pytest.mark.parametrize("prec", [0, 1])
-def test_ties_and_duplicates(prec):
+def test_ties_and_duplicates(prec, method):
     rng = np.random.default_rng(5)
     pts = rng.integers(0, 5, size=(3000, 3)).astype(np.float64)
     for cut in (0, None):
         exp = vo.knn(pts, pts, None, 24, ref_cut=cut)
         np.testing.assert_array_equal(hip_knn(pts, pts, None, 24, cut, prec), exp)
+        np.testing.assert_array_equal(hip_knn_self(pts, None, 24, cut, prec), exp)
     ones = np.ones((70, 3))
     exp = vo.knn(ones, ones, 1.0, 16, ref_cut=0)
     np.testing.assert_array_equal(hip_knn(ones, ones, 1.0, 16, 0, prec), exp)
 
 
 @pytest.mark.parametrize("prec", [0, 1])
-def test_empty_inputs(prec):
+def test_empty_inputs(prec, method):
     x = np.random.default_rng(0).random((10, 3))
     out = hip_ops.knn(t(x), t(np.zeros((0, 3))), None, 4, precision=prec)
     assert out.shape == (0, 4)
@@ -84,10 +101,82 @@ def test_empty_inputs(prec):
 
 
 @pytest.mark.parametrize("prec", [0, 1])
-def test_road_mixture_slice(prec):
+def test_road_mixture_slice(prec, method):
     ds = datasets.road_mixture(n_train=60_000, n_test=4_000)
     x, xq = ds["x_train"], ds["x_test"]
     exp = vo.knn(x, x, 0.1, 32, ref_cut=0)
     np.testing.assert_array_equal(hip_knn(x, x, 0.1, 32, 0, prec), exp)
+    np.testing.assert_array_equal(hip_knn_self(x, 0.1, 32, 0, prec), exp)
     exp = vo.knn(x, xq, 0.1, 32)
     np.testing.assert_array_equal(hip_knn(x, xq, 0.1, 32, None, prec), exp)
+
+
+@pytest.mark.parametrize("d", [1, 2, 3])
+@pytest.mark.parametrize("cut", [None, 0, 5])
+def test_non_finite_coordinates(d, cut, method):
+    """Refs with a NaN or infinite coordinate are never neighbours; a query with one
+    gets -1 only (the scans' admission never takes a NaN or infinite key)."""
+    rng = np.random.default_rng(40 + d)
+    x = np.round(rng.normal(size=(3000, d)), 2)
+    x[[3, 50, 777], 0] = np.nan
+    x[[9, 1500], d - 1] = np.inf
+    x[2222, 0] = -np.inf
+    q = x if cut is not None else np.concatenate([x[:50], rng.normal(size=(50, d))])
+    exp = vo.knn(x, q, None, 17, ref_cut=cut)
+    np.testing.assert_array_equal(hip_knn(x, q, None, 17, cut, 1), exp)
+    if cut is not None:
+        np.testing.assert_array_equal(hip_knn_self(x, None, 17, cut, 0), exp)
+
+
+@pytest.mark.parametrize("shape", ["cluster", "line", "plane", "point", "far_queries", "spread"])
+def test_point_set_shapes(shape, method):
+    """Clustered, lower-dimensional and degenerate point sets, and queries far
+    outside the refs' bounding box."""
+    rng = np.random.default_rng(11)
+    n = 20_000
+    if shape == "cluster":
+        x = np.concatenate([rng.normal(size=(n // 2, 3)) * 1e-4, rng.normal(size=(n // 2, 3)) * 10])
+        rng.shuffle(x)
+    elif shape == "line":
+        s_ = rng.random(n)
+        x = np.stack([s_, 2 * s_ + 1, np.full(n, 0.5)], 1)
+    elif shape == "plane":
+        x = np.concatenate([rng.random((n, 2)), np.zeros((n, 1))], 1)
+    elif shape == "point":
+        x = np.full((n // 10, 3), 2.5)
+    elif shape == "spread":
+        x = rng.random((n, 3)) * np.array([1e6, 1.0, 1e-6]) + np.array([3e7, 0.0, -1.0])
+    else:
+        x = rng.random((n, 3))
+    q = rng.normal(size=(500, 3)) * 50 if shape == "far_queries" else x[:700] + 1e-3
+    for cut, qq in ((0, x), (None, q)):
+        exp = vo.knn(x, qq, None, 32, ref_cut=cut)
+        got = hip_knn_self(x, None, 32, 0, 1) if cut == 0 else hip_knn(x, qq, None, 32, None, 1)
+        np.testing.assert_array_equal(got, exp)
+
+
+@pytest.mark.parametrize("k", [1, 16, 33, 64])
+def test_k_beyond_refs_and_small_sets(k, method):
+    rng = np.random.default_rng(k)
+    for n in (1, 5, 40, 130):
+        x = rng.random((n, 3))
+        q = rng.random((9, 3))
+        np.testing.assert_array_equal(hip_knn(x, q, 0.5, k, None, 0), vo.knn(x, q, 0.5, k))
+        np.testing.assert_array_equal(hip_knn_self(x, 0.5, k, 0, 1), vo.knn(x, x, 0.5, k, ref_cut=0))
+
+
+def test_methods_agree_on_road_mixture_prefix():
+    """Grid search and brute-force scans on the first 150,000 road-mixture rows
+    (self-search, Vecchia cut 0, m = 32): identical indices."""
+    ds = datasets.road_mixture()
+    x = ds["x_train"][:150_000]
+    outs = []
+    for m in METHODS:
+        with hip_ops.knn_method(m):
+            outs.append(hip_knn_self(x, 0.1, 32, 0, 1))
+    np.testing.assert_array_equal(outs[0], outs[1])
+    # a sample of rows against the oracle on each row's own prefix of refs
+    rows = np.concatenate([[0, 1, 31, 32, 33], np.random.default_rng(0).choice(x.shape[0], 40)])
+    for r in rows:
+        e = vo.knn(x[:r], x[r:r + 1], 0.1, 32)[0] if r > 0 else -np.ones(32, np.int32)
+        np.testing.assert_array_equal(outs[0][r], e)

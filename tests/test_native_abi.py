@@ -54,3 +54,15 @@ def test_error_mapping():
     import numpy as np
     with pytest.raises(np.linalg.LinAlgError):
         nat.check(nat.NNGP_ERR_NOT_POSITIVE_DEFINITE)
+
+
+def test_knn_method_setting():
+    """The search-method switch is host state only (no GPU call)."""
+    assert nat.knn_get_method() == nat.KNN_AUTO
+    nat.knn_set_method(nat.KNN_BRUTE_FORCE)
+    assert nat.knn_get_method() == nat.KNN_BRUTE_FORCE
+    nat.knn_set_method(nat.KNN_AUTO)
+    assert nat.knn_get_method() == nat.KNN_AUTO
+    with pytest.raises(ValueError):
+        nat.knn_set_method(7)
+    assert nat.knn_get_method() == nat.KNN_AUTO
