@@ -51,25 +51,34 @@ struct GridRefMem {
     GridMem full;
 };
 
-// Per search: a grid over a prefix of the refs (lower Vecchia bands) and the
-// queries of one band sorted by tile.
+// Query buckets of one band (tile order over the band's grid).
+struct GridQueryBins {
+    int32_t* cnt;
+    int32_t* start;
+    int32_t* fill;
+    int32_t* sums;
+    int capq;
+};
+
+constexpr int kMaxGridBands = 28;
+
+// Per search: grids over prefixes of the refs (the lower Vecchia bands, band j
+// holding refs [0, 2 L0 2^j)), the query buckets of every band (band n_lower is
+// the top band, searched on the full grid; without a cut it is the only band),
+// and the queries sorted band by band.
 struct GridSearchMem {
-    GridMem band;
-    int32_t* qcnt;
-    int32_t* qstart;
-    int32_t* qfill;
-    int32_t* qsums;
+    int n_lower;
+    GridMem band[kMaxGridBands];
+    GridQueryBins qb[kMaxGridBands + 1];
     int32_t* qcell_of;
     GridPt* qpts;
-    int32_t* qitem;   // tile of each work item (64 queries of one tile)
-    int capq;
 };
 
 bool grid_supported(int d, int k);
 // Vecchia rows whose ref limit is below this are scanned brute force.
 int64_t grid_prefix_limit(int64_t n_ref, int k);
 void carve_grid_refs(Carver& c, int64_t n_ref, GridRefMem* M);
-void carve_grid_search(Carver& c, int64_t n_ref, int64_t n_query, bool with_cut,
+void carve_grid_search(Carver& c, int64_t n_ref, int64_t n_query, int k, bool with_cut,
                        GridSearchMem* M);
 // Builds the grid over n scaled points (u64, n x d AoS).
 int grid_build(const double* u64, int64_t n, int d, const GridRefMem& M, hipStream_t st);

@@ -818,7 +818,7 @@ QueryMem carve_query(Carver& c, int64_t n_ref, int64_t n_query, int d, int k, in
     Q.lkey = c.take<double>((size_t)k * kFallbackThreads);
     Q.lidx = c.take<int32_t>((size_t)k * kFallbackThreads);
     if (grid_supported(d, k)) {
-        carve_grid_search(c, n_ref, n_query, with_cut, &Q.grid);
+        carve_grid_search(c, n_ref, n_query, k, with_cut, &Q.grid);
         if (with_cut) {
             const int64_t pre = round_up(grid_prefix_limit(n_ref, k), 64) + 64;
             Q.pre_r = c.take<double>((size_t)pre * scan_dim(d));

@@ -326,56 +326,26 @@ void bin_points(const double* u64, int64_t n, int d, int64_t idx_off, const Grid
 // ---------------------------------------------------------------------------
 // Search kernel
 // ---------------------------------------------------------------------------
-struct CoopArgs {
+// One band of a search: its grid and its queries (tile order); its waves are
+// [wave_off, wave_off + ceil(n_q / 64)).
+struct BandDesc {
     const GridParams* gp;
     const int32_t* cell_start;  // ref cells, row-major
     const GridPt* rpts;         // refs in cell order
     const GridPt* qpts;         // the band's queries in tile order
-    const int32_t* qstart;      // start of each query bucket (capq + 2 entries)
-    const int32_t* wp;          // first work item of each tile; wp[capq + 1] = item count
-    const int32_t* item_tile;   // tile of each work item
-    int capq;
+    int64_t n_q;
+    int64_t wave_off;
+};
+
+struct SearchArgs {
+    int n_bands;
+    BandDesc band[kMaxGridBands + 1];
     int64_t n_ref;
     int has_cut;
     int64_t cut;
     int k;
     int32_t* out_idx;
 };
-
-// Queries [b, e) of tile t in the sorted query array; tile n_tiles stands for the
-// bucket of queries with a non-finite coordinate.
-__device__ __forceinline__ void tile_range(const GridParams& P, const int32_t* qstart, int capq,
-                                           int t, int32_t& b, int32_t& e) {
-    const int tv = P.t[0] * P.t[1] * P.t[2];
-    const int nt = P.n_qb / tv;
-    if (t < nt) {
-        b = qstart[t * tv];
-        e = qstart[(t + 1) * tv];
-    } else if (t == nt) {
-        b = qstart[capq];
-        e = qstart[capq + 1];
-    } else {
-        b = e = 0;
-    }
-}
-
-// Work items of each tile: its queries in groups of 64 (one wave each).
-__global__ void items_kernel(const GridParams* gp, const int32_t* qstart, int capq,
-                             int32_t* items) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= capq + 2) return;
-    int32_t b, e;
-    tile_range(*gp, qstart, capq, t, b, e);
-    items[t] = (e - b + 63) / 64;
-}
-
-__global__ void expand_items_kernel(const int32_t* items, const int32_t* wp, int n_tiles,
-                                    int32_t* item_tile) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n_tiles) return;
-    const int32_t c = items[t], o = wp[t];
-    for (int32_t i = 0; i < c; ++i) item_tile[o + i] = t;
-}
 
 // Sorted list of K entries, the first K - k of them fixed at (-inf, -2): the
 // last entry is then the k-th smallest admitted pair.
@@ -395,88 +365,99 @@ __device__ __forceinline__ void store_list(const int (&ids)[K], int pad, int32_t
         if (t >= pad) o[t - pad] = ids[t];
 }
 
-// Value of v in lane src (wave-uniform result).
-__device__ __forceinline__ double read_lane_f64(double v, int src) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), src);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-
+// One lane per query, 64 neighbouring queries per wave.  A lane walks the rings
+// of cells around its own cell (R = 0, 1, ...; a ring's rows (y, z) one after the
+// other: a new row's whole x run, an old row's two new end cells) and appends the
+// refs below its admission threshold to its LDS column; when any lane's column is
+// full, or a lane finished a ring, the wave merges every column into the lanes'
+// sorted register lists in lock-step (the single merge site of the loop).  After a
+// ring the lane stops once its k-th key is below the squared distance from its
+// query to the nearest cell outside the visited block.
 template <int D, int K>
 __global__ __launch_bounds__(64 * kCoopWaves)
 __attribute__((amdgpu_waves_per_eu(K <= 16 ? 3 : 2)))
-void coop_knn_kernel(CoopArgs a) {
+void lane_knn_kernel(SearchArgs a) {
     constexpr int B = kCoopSlots;
-    constexpr int CH = 4;
+    constexpr int CH = 4;  // refs per step of a lane
     __shared__ double sk[kCoopWaves][B][64];
     __shared__ int si[kCoopWaves][B][64];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
     const int lane = threadIdx.x & 63;
-    const GridParams& P = *a.gp;
-    const int64_t n_items = a.wp[a.capq + 1];
-    const int64_t n_waves = (int64_t)gridDim.x * kCoopWaves;
-    double (*bk)[64] = sk[w];
-    int (*bi)[64] = si[w];
-#pragma unroll 1
-    for (int64_t item = (int64_t)blockIdx.x * kCoopWaves + w; item < n_items; item += n_waves) {
-        const int t = a.item_tile[item];
-        int32_t tb, te;
-        tile_range(P, a.qstart, a.capq, t, tb, te);
-        const int64_t j = (int64_t)tb + 64 * (int64_t)(item - a.wp[t]) + lane;
-        const bool jv = j < te;
-        double qu[3] = {0.0, 0.0, 0.0};
-        int64_t q = 0;
-        if (jv) {
-            const GridPt qp = a.qpts[j];
+    const int64_t wg = (int64_t)blockIdx.x * kCoopWaves + w;
+    int bj = 0;
+    for (int t = 1; t < a.n_bands; ++t)
+        if (a.band[t].wave_off <= wg) bj = t;
+    const BandDesc& bd0 = a.band[bj];
+    const int64_t j = (wg - bd0.wave_off) * 64 + lane;
+    const bool jv = j < bd0.n_q;
+    double qu[3] = {0.0, 0.0, 0.0};
+    int64_t q = 0;
+    if (jv) {
+        const GridPt qp = bd0.qpts[j];
 #pragma unroll
-            for (int f = 0; f < 3; ++f) qu[f] = qp.u[f];
-            q = qp.idx;
+        for (int f = 0; f < 3; ++f) qu[f] = qp.u[f];
+        q = qp.idx;
+    }
+    int64_t lim = 0;
+    if (jv) {
+        lim = a.n_ref;
+        if (a.has_cut) {
+            lim = q + a.cut;
+            lim = lim < 0 ? 0 : (lim > a.n_ref ? a.n_ref : lim);
         }
-        int64_t lim = 0;
-        if (jv) {
-            lim = a.n_ref;
-            if (a.has_cut) {
-                lim = q + a.cut;
-                lim = lim < 0 ? 0 : (lim > a.n_ref ? a.n_ref : lim);
-            }
-        }
-        bool fin = jv;
+    }
+    bool fin = jv;
 #pragma unroll
-        for (int f = 0; f < D; ++f) fin = fin && __builtin_isfinite(qu[f]);
-        const bool active = fin && lim > 0;
-        int g[3], c[3];
-        double lo[3];
+    for (int f = 0; f < D; ++f) fin = fin && __builtin_isfinite(qu[f]);
+    const bool active = fin && lim > 0;
+    const GridParams& P = *bd0.gp;
+    int g[3], c[3];
+    double lo[3];
 #pragma unroll
-        for (int f = 0; f < 3; ++f) {
-            g[f] = P.g[f];
-            lo[f] = P.lo[f];
-            c[f] = (f < D && active) ? cell_coord(qu[f], lo[f], P.inv_h, g[f]) : 0;
-        }
-        // the wave's box of query cells
-        int bmin[3], bmax[3];
-#pragma unroll
-        for (int f = 0; f < 3; ++f) {
-            bmin[f] = wave_min32(active ? c[f] : INT_MAX);
-            bmax[f] = wave_max32(active ? c[f] : INT_MIN);
-        }
-        // K - k leading placeholder entries; laundered through a register so that
-        // the per-entry masks are rebuilt here and at the store, not kept live
-        int pad = K - a.k;
-        asm volatile("" : "+v"(pad));
-        double keys[K];
-        int ids[K];
-        init_list<K>(keys, ids, pad);
+    for (int f = 0; f < 3; ++f) {
+        g[f] = P.g[f];
+        lo[f] = P.lo[f];
+        c[f] = (f < D && active) ? cell_coord(qu[f], lo[f], P.inv_h, g[f]) : 0;
+    }
+    // K - k leading placeholder entries; laundered through a register so that
+    // the per-entry masks are rebuilt here and at the store, not kept live
+    int pad = K - a.k;
+    asm volatile("" : "+v"(pad));
+    double keys[K];
+    int ids[K];
+    init_list<K>(keys, ids, pad);
 
-        if (__builtin_amdgcn_ballot_w64(active)) {
-            const double h = P.h, tol = P.tol;
-            const int32_t* cs = a.cell_start;
-            const int32_t lim32 = (int32_t)lim;
-            // admission threshold pair; inactive lanes admit nothing
-            double tk = active ? __builtin_inf() : -__builtin_inf();
-            int ti = -1;
-            int cnt = 0;
-            auto merge = [&]() {
+    if (__builtin_amdgcn_ballot_w64(active)) {
+        const double h = P.h, tol = P.tol;
+        const int32_t* cs = bd0.cell_start;
+        const GridPt* rpts = bd0.rpts;
+        const int32_t lim32 = (int32_t)lim;
+        double (*bk)[64] = sk[w];
+        int (*bi)[64] = si[w];
+        // admission threshold pair; inactive lanes admit nothing
+        double tk = active ? __builtin_inf() : -__builtin_inf();
+        int ti = -1;
+        int cnt = 0;
+        // per-lane walk: ring R, segment sg of the ring (two per row), refs [p, e)
+        int R = 0, sg = 0, nseg = 0, ny = 1;
+        int x0 = 0, x1 = 0, y0 = 0, z0 = 0;
+        auto ring_bounds = [&]() {
+            x0 = max(c[0] - R, 0);
+            x1 = min(c[0] + R, g[0] - 1);
+            y0 = max(c[1] - R, 0);
+            const int y1 = min(c[1] + R, g[1] - 1);
+            z0 = max(c[2] - R, 0);
+            const int z1 = min(c[2] + R, g[2] - 1);
+            ny = y1 - y0 + 1;
+            nseg = 2 * ny * (z1 - z0 + 1);
+        };
+        ring_bounds();
+        int32_t p = 0, e = 0;
+        bool ring_end = false;   // ring walked: wait for the merge, then the stop test
+        bool done = !active;
+#pragma unroll 1
+        for (;;) {
+            if (__builtin_amdgcn_ballot_w64((cnt > B - CH) | (ring_end & (cnt > 0)))) {
                 const int nb = wave_max32(cnt);
 #pragma unroll 1
                 for (int b = 0; b < nb; ++b) {
@@ -490,198 +471,112 @@ void coop_knn_kernel(CoopArgs a) {
                     tk = keys[K - 1];
                     ti = ids[K - 1];
                 }
-            };
-            // The traversal is one flat loop (rings -> batches of 64 rows -> rounds
-            // -> groups of CH refs) so that the list merge has a single call site.
-            int R = 0, rb = 0, nrows = 0, ny = 1;
-            int x0 = 0, x1 = 0, y0 = 0, y1 = 0, z0 = 0, z1 = 0;
-            auto ring_bounds = [&]() {
-                x0 = max(bmin[0] - R, 0);
-                x1 = min(bmax[0] + R, g[0] - 1);
-                y0 = max(bmin[1] - R, 0);
-                y1 = min(bmax[1] + R, g[1] - 1);
-                z0 = max(bmin[2] - R, 0);
-                z1 = min(bmax[2] + R, g[2] - 1);
-                ny = y1 - y0 + 1;
-                nrows = ny * (z1 - z0 + 1);
-            };
-            ring_bounds();
-            bool need_batch = true;  // set up the next 64 rows of the ring
-            bool flush = false;      // ring finished: merge, then stop test
-            // the batch's rows, one per lane: refs [sA, sA + lA) then [sB, ...), and
-            // the exclusive prefix `off` of the row lengths (the batch's ref stream)
-            int32_t sA = 0, lA = 0, sB = 0, off = 0, tot = 0, t0 = 0;
-            GridPt nxt, cp;
-            uint64_t m = 0;          // lanes whose round ref is still to be scored
-            // Ref at stream position t of the current batch: the row holding it by a
-            // binary search over the lanes' prefixes.  Called with every lane active
-            // (the shuffles read the other lanes' registers).
-            auto ref_at = [&](int32_t t) -> int32_t {
-                int j = 0;
+            }
+            if (ring_end) {
+                // Lower bound of the distance from the query to any ref in a cell
+                // outside the block walked so far: the nearest block face with
+                // cells beyond it.
+                double bd = __builtin_inf();
 #pragma unroll
-                for (int step = 32; step > 0; step >>= 1) {
-                    const int c2 = j + step;
-                    if (__shfl(off, c2, 64) <= t) j = c2;
+                for (int f = 0; f < D; ++f) {
+                    if (c[f] - R >= 1) bd = fmin(bd, qu[f] - (lo[f] + (double)(c[f] - R) * h));
+                    if (c[f] + R + 1 <= g[f] - 1)
+                        bd = fmin(bd, (lo[f] + (double)(c[f] + R + 1) * h) - qu[f]);
                 }
-                // all shuffles unconditionally: a shuffle under a per-lane branch
-                // would read lanes that did not take that branch
-                const int32_t loc = t - __shfl(off, j, 64);
-                const int32_t la = __shfl(lA, j, 64);
-                const int32_t sa = __shfl(sA, j, 64);
-                const int32_t sb = __shfl(sB, j, 64);
-                return loc < la ? sa + loc : sb + (loc - la);
-            };
-#pragma unroll 1
-            for (;;) {
-                if (__builtin_amdgcn_ballot_w64((cnt > B - CH) | (flush & (cnt > 0)))) merge();
-                if (flush) {
-                    // Lower bound of the distance from the query to any ref in a cell
-                    // outside the visited box: the nearest box face with cells beyond.
-                    double bd = __builtin_inf();
-#pragma unroll
-                    for (int f = 0; f < D; ++f) {
-                        if (bmin[f] - R >= 1)
-                            bd = fmin(bd, qu[f] - (lo[f] + (double)(bmin[f] - R) * h));
-                        if (bmax[f] + R + 1 <= g[f] - 1)
-                            bd = fmin(bd, (lo[f] + (double)(bmax[f] + R + 1) * h) - qu[f]);
-                    }
-                    bool done = !active || bd == __builtin_inf();  // infinite: all visited
-                    if (!done) {
-                        const double bb = bd - tol;
-                        done = bb > 0.0 && keys[K - 1] < bb * bb;
-                    }
-                    if (!__builtin_amdgcn_ballot_w64(!done)) break;
+                done = bd == __builtin_inf();  // every cell visited
+                if (!done) {
+                    const double bb = bd - tol;
+                    done = bb > 0.0 && keys[K - 1] < bb * bb;
+                }
+                ring_end = false;
+                if (!done) {
                     ++R;
                     ring_bounds();
-                    rb = 0;
-                    flush = false;
-                    need_batch = true;
-                    continue;
-                }
-                if (need_batch) {
-                    if (rb >= nrows) {
-                        flush = true;
-                        continue;
-                    }
-                    // Rows (y, z) of the ring, one per lane: a new row contributes
-                    // its whole x run, an old row its two new end cells.
-                    const int r = rb + lane;
-                    rb += 64;
-                    sA = 0;
-                    lA = 0;
-                    sB = 0;
-                    int32_t lB = 0;
-                    if (r < nrows) {
-                        const int y = y0 + r % ny, z = z0 + r / ny;
-                        const int base = g[0] * (y + g[1] * z);
-                        const bool full = R == 0 || z == bmin[2] - R || z == bmax[2] + R ||
-                                          y == bmin[1] - R || y == bmax[1] + R;
-                        if (full) {
-                            sA = cs[base + x0];
-                            lA = cs[base + x1 + 1] - sA;
-                        } else {
-                            const int xa = bmin[0] - R, xb = bmax[0] + R;
-                            if (xa >= 0) {
-                                sA = cs[base + xa];
-                                lA = cs[base + xa + 1] - sA;
-                            }
-                            if (xb < g[0]) {
-                                sB = cs[base + xb];
-                                lB = cs[base + xb + 1] - sB;
-                            }
-                        }
-                    }
-                    // exclusive prefix of the row lengths over the wave
-                    const int32_t len = lA + lB;
-                    int32_t inc = len;
-#pragma unroll
-                    for (int o = 1; o < 64; o <<= 1) {
-                        const int32_t y2 = __shfl_up(inc, o, 64);
-                        if (lane >= o) inc += y2;
-                    }
-                    off = inc - len;
-                    tot = __builtin_amdgcn_readlane(inc, 63);
-                    t0 = 0;
-                    need_batch = false;
-                    m = 0;
-                    {
-                        // every lane runs the search (the shuffles read all lanes)
-                        const int32_t pi = ref_at(min(lane, max(tot - 1, 0)));
-                        if (lane < tot) nxt = a.rpts[pi];
-                    }
-                }
-                if (m == 0) {
-                    // next round: 64 consecutive refs of the batch's stream, one per
-                    // lane (the following round is prefetched)
-                    if (t0 >= tot) {
-                        need_batch = true;
-                        continue;
-                    }
-                    m = __builtin_amdgcn_ballot_w64(t0 + lane < tot);
-                    cp = nxt;
-                    t0 += 64;
-                    if (t0 < tot) {
-                        const int32_t pi = ref_at(min(t0 + lane, tot - 1));
-                        if (t0 + lane < tot) nxt = a.rpts[pi];
-                    }
-                }
-                // score up to CH refs of the round, read from the lanes holding them
-                double key[CH];
-                int rid[CH];
-                bool hit[CH];
-                bool any = false;
-#pragma unroll
-                for (int u = 0; u < CH; ++u) {
-                    const bool ok = m != 0;
-                    const int src = ok ? (int)__builtin_ctzll(m) : 0;
-                    m &= m - 1;
-                    const double t0 = read_lane_f64(cp.u[0], src) - qu[0];
-                    key[u] = t0 * t0;
-#pragma unroll
-                    for (int f = 1; f < D; ++f) {
-                        const double tt = read_lane_f64(cp.u[f], src) - qu[f];
-                        key[u] = fma(tt, tt, key[u]);
-                    }
-                    rid[u] = __builtin_amdgcn_readlane((int)cp.idx, src);
-                    hit[u] = ok & (rid[u] < lim32) &
-                             ((key[u] < tk) | ((key[u] == tk) & (rid[u] < ti)));
-                    any = any | hit[u];
-                }
-                if (any) {
-#pragma unroll
-                    for (int u = 0; u < CH; ++u) {
-                        if (hit[u]) {
-                            bk[cnt][lane] = key[u];
-                            bi[cnt][lane] = rid[u];
-                            ++cnt;
-                        }
-                    }
+                    sg = 0;
                 }
             }
+            if (!__builtin_amdgcn_ballot_w64(!done)) break;
+            if (done) continue;
+            if (p < e) {
+                // the next CH refs of the segment, loaded together
+                GridPt rp[CH];
+#pragma unroll
+                for (int u = 0; u < CH; ++u) rp[u] = rpts[min(p + u, e - 1)];
+#pragma unroll
+                for (int u = 0; u < CH; ++u) {
+                    const double t0 = rp[u].u[0] - qu[0];
+                    double key = t0 * t0;
+#pragma unroll
+                    for (int f = 1; f < D; ++f) {
+                        const double tt = rp[u].u[f] - qu[f];
+                        key = fma(tt, tt, key);
+                    }
+                    const int ri = (int)rp[u].idx;
+                    if ((p + u < e) & (ri < lim32) &
+                        ((key < tk) | ((key == tk) & (ri < ti)))) {
+                        bk[cnt][lane] = key;
+                        bi[cnt][lane] = ri;
+                        ++cnt;
+                    }
+                }
+                p += CH;
+            } else if (sg < nseg) {
+                // next segment of the ring
+                const int r = sg >> 1, part = sg & 1;
+                ++sg;
+                const int y = y0 + r % ny, z = z0 + r / ny;
+                const int base = g[0] * (y + g[1] * z);
+                const bool full = R == 0 || z == c[2] - R || z == c[2] + R || y == c[1] - R ||
+                                  y == c[1] + R;
+                int cx0 = 1, cx1 = 0;  // empty
+                if (full) {
+                    if (part == 0) {
+                        cx0 = x0;
+                        cx1 = x1;
+                    }
+                } else {
+                    const int xc = part == 0 ? c[0] - R : c[0] + R;
+                    if (xc >= 0 && xc < g[0]) cx0 = cx1 = xc;
+                }
+                if (cx0 <= cx1) {
+                    p = cs[base + cx0];
+                    e = cs[base + cx1 + 1];
+                }
+            } else {
+                ring_end = true;
+            }
         }
-        int pad_out = K - a.k;
-        asm volatile("" : "+v"(pad_out));
-        if (jv) store_list<K>(ids, pad_out, a.out_idx + q * a.k);
     }
+    int pad_out = K - a.k;
+    asm volatile("" : "+v"(pad_out));
+    if (jv) store_list<K>(ids, pad_out, a.out_idx + q * a.k);
 }
 
 template <int D, int K>
-void launch_coop_dk(const CoopArgs& c, unsigned blocks, hipStream_t st) {
-    hipLaunchKernelGGL((coop_knn_kernel<D, K>), dim3(blocks), dim3(64 * kCoopWaves), 0, st, c);
+void launch_search_dk(const SearchArgs& c, unsigned blocks, hipStream_t st) {
+    hipLaunchKernelGGL((lane_knn_kernel<D, K>), dim3(blocks), dim3(64 * kCoopWaves), 0, st, c);
 }
 
-void launch_coop(const CoopArgs& c, int d, unsigned blocks, hipStream_t st) {
+// One launch over every band's waves.
+void launch_search(SearchArgs& c, int d, hipStream_t st) {
+    int64_t waves = 0;
+    for (int t = 0; t < c.n_bands; ++t) {
+        c.band[t].wave_off = waves;
+        waves += (c.band[t].n_q + 63) / 64;
+    }
+    if (waves == 0) return;
+    const unsigned blocks = (unsigned)((waves + kCoopWaves - 1) / kCoopWaves);
     const int K = c.k <= 16 ? 16 : (c.k <= 32 ? 32 : 64);
     switch (d * 100 + K) {
-        case 116: launch_coop_dk<1, 16>(c, blocks, st); break;
-        case 132: launch_coop_dk<1, 32>(c, blocks, st); break;
-        case 164: launch_coop_dk<1, 64>(c, blocks, st); break;
-        case 216: launch_coop_dk<2, 16>(c, blocks, st); break;
-        case 232: launch_coop_dk<2, 32>(c, blocks, st); break;
-        case 264: launch_coop_dk<2, 64>(c, blocks, st); break;
-        case 316: launch_coop_dk<3, 16>(c, blocks, st); break;
-        case 332: launch_coop_dk<3, 32>(c, blocks, st); break;
-        default: launch_coop_dk<3, 64>(c, blocks, st); break;
+        case 116: launch_search_dk<1, 16>(c, blocks, st); break;
+        case 132: launch_search_dk<1, 32>(c, blocks, st); break;
+        case 164: launch_search_dk<1, 64>(c, blocks, st); break;
+        case 216: launch_search_dk<2, 16>(c, blocks, st); break;
+        case 232: launch_search_dk<2, 32>(c, blocks, st); break;
+        case 264: launch_search_dk<2, 64>(c, blocks, st); break;
+        case 316: launch_search_dk<3, 16>(c, blocks, st); break;
+        case 332: launch_search_dk<3, 32>(c, blocks, st); break;
+        default: launch_search_dk<3, 64>(c, blocks, st); break;
     }
 }
 
@@ -715,48 +610,28 @@ void build_grid(const double* u64, int64_t n_pts, int d, const long long* bbox, 
                M.cell_of, M.pts, st);
 }
 
-// Searches query rows [qa, qb) against grid M, which holds every ref the rows
-// may reach.
-void search_band(const GridMem& M, const double* qu64, int64_t qa, int64_t qb, int d, int k,
-                 int has_cut, int64_t cut, int64_t n_ref, const GridSearchMem& S, int32_t* out_idx,
-                 hipStream_t st) {
-    const int64_t nq = qb - qa;
-    if (nq <= 0) return;
-    const int64_t nt = (int64_t)S.capq + 2;  // tiles (bound) + non-finite bucket + total
-    timing_begin(TK_KNN_GRID_BUILD, st);
-    bin_points(qu64 + qa * d, nq, d, qa, M.params, 1, S.capq + 1, S.qcnt, S.qstart, S.qfill,
-               S.qsums, S.qcell_of, S.qpts, st);
-    // work items: qcnt <- items per tile, qfill <- their exclusive prefix
-    const unsigned gt = (unsigned)((nt + 255) / 256);
-    hipLaunchKernelGGL(items_kernel, dim3(gt), dim3(256), 0, st, M.params, S.qstart, S.capq,
-                       S.qcnt);
-    const int64_t nsb = scan_blocks(nt);
-    hipLaunchKernelGGL(scan_local_kernel, dim3((unsigned)nsb), dim3(256), 0, st, S.qcnt, nt,
-                       S.qfill, S.qsums);
-    hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(256), 0, st, S.qsums, (int)nsb);
-    hipLaunchKernelGGL(scan_add_kernel, dim3((unsigned)nsb), dim3(256), 0, st, S.qfill, nt, S.qsums);
-    hipLaunchKernelGGL(expand_items_kernel, dim3(gt), dim3(256), 0, st, S.qcnt, S.qfill, (int)nt,
-                       S.qitem);
-    timing_end(TK_KNN_GRID_BUILD, st);
-    CoopArgs c{};
-    c.gp = M.params;
-    c.cell_start = M.start;
-    c.rpts = M.pts;
-    c.qpts = S.qpts;
-    c.qstart = S.qstart;
-    c.wp = S.qfill;
-    c.item_tile = S.qitem;
-    c.capq = S.capq;
-    c.n_ref = n_ref;
-    c.has_cut = has_cut;
-    c.cut = cut;
-    c.k = k;
-    c.out_idx = out_idx;
-    // waves stride over the work items (their count is known on the device only)
-    const int64_t waves = std::min<int64_t>(8192, std::max<int64_t>(4, (nq + 31) / 32));
-    timing_begin(TK_KNN_GRID, st);
-    launch_coop(c, d, (unsigned)((waves + kCoopWaves - 1) / kCoopWaves), st);
-    timing_end(TK_KNN_GRID, st);
+// Sorts query rows [qa, qb) by tile of grid M into `bins` and qpts[pos ...] and
+// describes the band.
+BandDesc bin_band(const GridMem& M, const double* qu64, int64_t qa, int64_t qb, int64_t pos, int d,
+                  const GridQueryBins& bins, const GridSearchMem& S, hipStream_t st) {
+    BandDesc b{};
+    b.gp = M.params;
+    b.cell_start = M.start;
+    b.rpts = M.pts;
+    b.qpts = S.qpts + pos;
+    b.n_q = qb - qa;
+    bin_points(qu64 + qa * d, qb - qa, d, qa, M.params, 1, bins.capq + 1, bins.cnt, bins.start,
+               bins.fill, bins.sums, S.qcell_of + pos, S.qpts + pos, st);
+    return b;
+}
+
+void carve_bins(Carver& c, int capq, GridQueryBins* Q) {
+    const int64_t ns = (int64_t)capq + 2;
+    Q->capq = capq;
+    Q->cnt = c.take<int32_t>((size_t)ns);
+    Q->start = c.take<int32_t>((size_t)ns);
+    Q->fill = c.take<int32_t>((size_t)ns);
+    Q->sums = c.take<int32_t>((size_t)scan_blocks(ns) + 1);
 }
 
 }  // namespace
@@ -765,7 +640,7 @@ bool grid_supported(int d, int k) { return d >= 1 && d <= 3 && k >= 1 && k <= 64
 
 int64_t grid_prefix_limit(int64_t n_ref, int k) {
     return std::max<int64_t>(std::max<int64_t>(64, 2 * (int64_t)k),
-                             std::min<int64_t>(16384, n_ref / 16));
+                             std::min<int64_t>(2048, n_ref / 16));
 }
 
 void carve_grid_refs(Carver& c, int64_t n_ref, GridRefMem* M) {
@@ -773,22 +648,22 @@ void carve_grid_refs(Carver& c, int64_t n_ref, GridRefMem* M) {
     carve_grid(c, n_ref, &M->full);
 }
 
-void carve_grid_search(Carver& c, int64_t n_ref, int64_t n_query, bool with_cut,
+void carve_grid_search(Carver& c, int64_t n_ref, int64_t n_query, int k, bool with_cut,
                        GridSearchMem* M) {
     *M = GridSearchMem{};
-    // lower bands hold fewer than n_ref refs
-    if (with_cut) carve_grid(c, n_ref, &M->band);
-    const int capq = grid_capq(grid_cap(n_ref));
-    const int64_t ns = (int64_t)capq + 2;
-    M->capq = capq;
-    M->qcnt = c.take<int32_t>((size_t)ns);
-    M->qstart = c.take<int32_t>((size_t)ns);
-    M->qfill = c.take<int32_t>((size_t)ns);
-    M->qsums = c.take<int32_t>((size_t)scan_blocks(ns) + 1);
+    int nl = 0;
+    if (with_cut) {
+        // lower bands: refs [0, 2L) with 2L < n_ref
+        for (int64_t L = grid_prefix_limit(n_ref, k); 2 * L < n_ref && nl < kMaxGridBands; L *= 2) {
+            carve_grid(c, 2 * L, &M->band[nl]);
+            carve_bins(c, grid_capq(M->band[nl].cap), &M->qb[nl]);
+            ++nl;
+        }
+    }
+    M->n_lower = nl;
+    carve_bins(c, grid_capq(grid_cap(n_ref)), &M->qb[nl]);
     M->qcell_of = c.take<int32_t>((size_t)n_query + 1);
     M->qpts = c.take<GridPt>((size_t)n_query + 1);
-    // work items: at most one per 64 queries plus one per non-empty tile
-    M->qitem = c.take<int32_t>((size_t)(n_query + n_query / 64 + 2));
 }
 
 int grid_build(const double* u64, int64_t n, int d, const GridRefMem& M, hipStream_t st) {
@@ -809,32 +684,43 @@ int grid_search(const GridRefMem& G, const double* ru64, int64_t n_ref, int d,
                 const double* qu64, int64_t n_query, int k, int has_cut, int64_t cut,
                 int64_t row0, const GridSearchMem& M, int32_t* out_idx, hipStream_t st) {
     if (!grid_supported(d, k)) return NNGP_ERR_UNSUPPORTED;
+    SearchArgs sa{};
+    sa.n_ref = n_ref;
+    sa.has_cut = has_cut;
+    sa.cut = cut;
+    sa.k = k;
+    sa.out_idx = out_idx;
+    timing_begin(TK_KNN_GRID_BUILD, st);
     if (!has_cut) {
-        search_band(G.full, qu64, 0, n_query, d, k, 0, 0, n_ref, M, out_idx, st);
-        NNGP_HIP_CHECK(hipGetLastError());
-        return NNGP_OK;
-    }
-    if (!M.band.params) return NNGP_ERR_WORKSPACE_TOO_SMALL;
-    // Band j: rows whose limit q + cut lies in [L, 2L), L = L0 * 2^j; the top band
-    // (2L >= n_ref) takes every row above and searches the full grid.
-    const int64_t L0 = grid_prefix_limit(n_ref, k);
-    for (int64_t L = L0;; L *= 2) {
-        const bool top = 2 * L >= n_ref;
-        int64_t qa = std::max<int64_t>(L - cut, row0);
-        int64_t qb = top ? n_query : std::min<int64_t>(2 * L - cut, n_query);
-        qa = std::min<int64_t>(qa, n_query);
-        if (qb > qa) {
-            const GridMem* gm = &G.full;
-            if (!top) {
-                timing_begin(TK_KNN_GRID_BUILD, st);
-                build_grid(ru64, 2 * L, d, G.bbox, M.band, st);
-                timing_end(TK_KNN_GRID_BUILD, st);
-                gm = &M.band;
+        sa.band[0] = bin_band(G.full, qu64, 0, n_query, 0, d, M.qb[M.n_lower], M, st);
+        sa.n_bands = 1;
+    } else {
+        // Band j: rows whose limit q + cut lies in [L, 2L), L = L0 * 2^j; the top
+        // band (2L >= n_ref) takes every row above and searches the full grid.
+        const int64_t L0 = grid_prefix_limit(n_ref, k);
+        int j = 0;
+        for (int64_t L = L0;; L *= 2, ++j) {
+            const bool top = 2 * L >= n_ref;
+            if (!top && j >= M.n_lower) return NNGP_ERR_WORKSPACE_TOO_SMALL;
+            int64_t qa = std::max<int64_t>(L - cut, row0);
+            int64_t qb = top ? n_query : std::min<int64_t>(2 * L - cut, n_query);
+            qa = std::min<int64_t>(qa, n_query);
+            if (qb > qa) {
+                const GridMem* gm = &G.full;
+                if (!top) {
+                    build_grid(ru64, 2 * L, d, G.bbox, M.band[j], st);
+                    gm = &M.band[j];
+                }
+                sa.band[sa.n_bands++] =
+                    bin_band(*gm, qu64, qa, qb, qa - row0, d, M.qb[top ? M.n_lower : j], M, st);
             }
-            search_band(*gm, qu64, qa, qb, d, k, 1, cut, n_ref, M, out_idx, st);
+            if (top) break;
         }
-        if (top) break;
     }
+    timing_end(TK_KNN_GRID_BUILD, st);
+    timing_begin(TK_KNN_GRID, st);
+    launch_search(sa, d, st);
+    timing_end(TK_KNN_GRID, st);
     NNGP_HIP_CHECK(hipGetLastError());
     return NNGP_OK;
 }
