@@ -161,7 +161,8 @@ int run_fused(const VecArgs& a, const VecLayout& L, int precision, hipStream_t s
         ex.flag_list = L.flag_list;
         ex.flag_count = L.flag_count;
         timing_begin(tk, st);
-        const int rc = launch_reg(a, ex, true, a.npts, st);
+        const int rc = seg_supported(a.m, a.d) ? launch_seg(a, ex, a.npts, st)
+                                               : launch_reg(a, ex, true, a.npts, st);
         timing_end(tk, st);
         if (rc) return NNGP_ERR_UNSUPPORTED;
         RegExtra redo{};

@@ -1,0 +1,472 @@
+// Fused FP64 Vecchia kernel for m <= 32, d <= 8 (gfx950, wave64): S = MR / 2 lanes
+// per neighbour block, G = 64 / S blocks per wave.
+//
+// Lane t of a block owns the two rows t and MR - 1 - t of the MR x MR neighbour
+// covariance (rows past m are padding and behave like the reference's -1 slots:
+// identity row/column, zero target), so every lane holds MR + 1 lower-triangle
+// entries and the factor's work is balanced over the lanes.
+//
+//   fill     each lane computes the FP64 RBF exponentials of its rows' lower
+//            triangle (and of the point itself); they stay in LDS for the
+//            adjoint, the covariance rows go to registers;
+//   factor   right-looking Cholesky: at step k every lane publishes its rows'
+//            column-k entries in LDS and reads the whole column back (one LDS
+//            round trip per step, no barriers beyond the wave);
+//   solves   forward substitution broadcasts z_k from the row's owner lane;
+//            backward substitution sums the lanes' partial products per step;
+//   reduce   sigma^2, mu, the log-likelihood and the adjoint gradient
+//            (dl/dtheta = g_s dc + dk.(a - 2 g_s b) + sum_ij dK_ij W_ij, each
+//            unordered pair (i, j) counted once with weight 2).
+// Same semantics as the generic and register kernels: no K or K_gradient ever
+// reaches HBM.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "common.h"
+#include "vecchia.h"
+#include "vecchia_reg.h"
+
+namespace nngp {
+
+namespace {
+
+constexpr double kHalfLog2PiS = 0.91893853320467274178;
+constexpr int kSegMaxDim = 8;
+// Pivot-ratio threshold of the FP32 form: blocks with max(diag) / min(pivot)
+// above it are re-factored in FP64.
+constexpr double kSegPivotRatioMax = 1.0e3;
+
+template <int S>
+__device__ __forceinline__ double seg_sum(double v) {
+#pragma unroll
+    for (int off = S / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, S);
+    return v;
+}
+
+// Validity of row j of this block: rows < S are bit j of vlo, rows >= S are bit
+// MR - 1 - j of vhi (the lanes' ballots of their first and second rows).
+template <int MR>
+__device__ __forceinline__ bool row_valid(int j, uint32_t vlo, uint32_t vhi) {
+    constexpr int S = MR / 2;
+    return j < S ? ((vlo >> j) & 1u) : ((vhi >> (MR - 1 - j)) & 1u);
+}
+
+__device__ __forceinline__ int tri(int i, int j) { return i * (i + 1) / 2 + j; }
+
+// Returns v unchanged but opaque to the compiler: comparisons of the lane's row
+// indices against an unrolled step index are then formed inside each step (and
+// consumed there) instead of being hoisted and kept live as lane masks.
+__device__ __forceinline__ int opaque(int v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
+template <typename T, int MR>
+__global__ __launch_bounds__(64) void vecchia_seg_kernel(VecArgs a, RegExtra ex) {
+    constexpr int S = MR / 2;
+    constexpr int G = 64 / S;
+    constexpr int TRI = MR * (MR + 1) / 2;
+    constexpr bool REFINE = sizeof(T) == 4;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x;
+    const int seg = lane / S;
+    const int t = lane % S;
+    const int i0 = t;               // first row of this lane
+    const int i1 = MR - 1 - t;      // second row
+    const int d = a.d;
+    const int ny = a.ny;
+    const KernParams& kp = a.kp;
+
+    // LDS per block (G blocks): FP64 covariance (lower triangle), two FP64
+    // vectors, X; then the factor's column broadcast and inverse diagonal (T).
+    double* Kt = reinterpret_cast<double*>(smem) + seg * TRI;
+    double* xv = reinterpret_cast<double*>(smem) + G * TRI + seg * MR;
+    double* vbuf = reinterpret_cast<double*>(smem) + G * (TRI + MR) + seg * MR;
+    double* X = reinterpret_cast<double*>(smem) + G * (TRI + 2 * MR) + seg * (MR + 1) * d;
+    T* col = reinterpret_cast<T*>(reinterpret_cast<double*>(smem) + G * (TRI + 2 * MR + (MR + 1) * d)) +
+             seg * 2 * MR;
+    T* dinv = col + MR;
+
+    // one group of G points per block
+    const int64_t count = ex.pcount ? (int64_t)*ex.pcount : a.npts;
+    const int64_t base = (int64_t)blockIdx.x * G;
+    if (base >= count) return;
+    const int64_t slot = base + seg;
+    const bool pv = slot < count;
+    const int64_t p = pv ? (ex.plist ? ex.plist[slot] : slot) : 0;
+
+    // ---------------- gather ---------------------------------------------
+    const int j0 = (pv && i0 < a.m) ? a.nn[p * a.m + i0] : -1;
+    const int j1 = (pv && i1 < a.m) ? a.nn[p * a.m + i1] : -1;
+    const bool v0 = j0 >= 0, v1 = j1 >= 0;
+    for (int f = 0; f < d; ++f) {
+        X[i0 * d + f] = v0 ? a.x[(int64_t)j0 * d + f] : 0.0;
+        X[i1 * d + f] = v1 ? a.x[(int64_t)j1 * d + f] : 0.0;
+    }
+    if (t == 0)
+        for (int f = 0; f < d; ++f) X[MR * d + f] = pv ? a.xs[p * d + f] : 0.0;
+    const uint64_t b0 = __ballot(v0), b1 = __ballot(v1);
+    const uint32_t vlo = (uint32_t)((b0 >> (seg * S)) & ((1ull << S) - 1ull));
+    const uint32_t vhi = (uint32_t)((b1 >> (seg * S)) & ((1ull << S) - 1ull));
+    __syncthreads();
+
+    // ---------------- fill: FP64 covariance of the lane's rows -------------
+    auto expo = [&](int i, int j) -> double {
+        const double* xi = X + i * d;
+        const double* xj = X + j * d;
+        double u = 0.0;
+        if (kp.n_ls == 1) {
+            for (int f = 0; f < d; ++f) {
+                const double tt = xi[f] - xj[f];
+                u = fma(tt, tt, u);
+            }
+            u *= kp.h[0];
+        } else {
+            for (int f = 0; f < d; ++f) {
+                const double tt = xi[f] - xj[f];
+                u = fma(tt * tt, kp.h[f], u);
+            }
+        }
+        return exp(-u);
+    };
+    const double diag_add = kp.w + a.alpha;
+    const double inv_s = kp.s != 0.0 ? 1.0 / kp.s : 0.0;
+    // exponential of a valid off-diagonal pair from its covariance entry
+    auto e_of = [&](double K) -> double { return kp.n_ls > 0 ? (K - kp.C) * inv_s : 0.0; };
+    // The lane's MR + 1 lower-triangle entries (i0, 0..i0), (i1, 0..i1), then its
+    // two rows against the point itself.
+    double e0s = 0.0, e1s = 0.0;
+#pragma unroll 1
+    for (int e = 0; e < MR + 3; ++e) {
+        int i, j;
+        bool vi;
+        if (e <= i0) {
+            i = i0;
+            j = e;
+            vi = v0;
+        } else if (e <= MR) {
+            i = i1;
+            j = e - i0 - 1;
+            vi = v1;
+        } else {
+            i = e == MR + 1 ? i0 : i1;
+            j = MR;
+            vi = (e == MR + 1 ? v0 : v1) && pv;
+        }
+        const bool vj = j == MR ? true : row_valid<MR>(j, vlo, vhi);
+        const double Ev = (vi && vj && kp.n_ls > 0 && i != j) ? expo(i, j) : (i == j ? 1.0 : 0.0);
+        if (j < MR) {
+            double K = (vi && vj) ? fma(kp.s, Ev, kp.C) : 0.0;
+            if (i == j) K = vi ? K + diag_add : 1.0;
+            Kt[tri(i, j)] = K;
+        } else if (e == MR + 1) {
+            e0s = Ev;
+        } else {
+            e1s = Ev;
+        }
+    }
+    const double k0 = v0 ? fma(kp.s, e0s, kp.C) : 0.0;
+    const double k1 = v1 ? fma(kp.s, e1s, kp.C) : 0.0;
+    const double c = kp.s + kp.C + kp.w + (a.alpha_self ? a.alpha : 0.0);
+    double dmax = fmax(v0 ? kp.s + kp.C + diag_add : 1.0, v1 ? kp.s + kp.C + diag_add : 1.0);
+#pragma unroll
+    for (int off = S / 2; off > 0; off >>= 1) dmax = fmax(dmax, __shfl_xor(dmax, off, S));
+    __syncthreads();
+
+    // ---------------- rows into registers (type T) --------------------------
+    // Entries right of a row's diagonal read neighbouring triangle entries: they
+    // are never used before being cleared below.
+    T L0[S], L1[MR];
+    {
+        const double* r0p = Kt + tri(i0, 0);
+        const double* r1p = Kt + tri(i1, 0);
+#pragma unroll
+        for (int j = 0; j < S; ++j) L0[j] = (T)r0p[j];
+#pragma unroll
+        for (int j = 0; j < MR; ++j) L1[j] = (T)r1p[j];
+    }
+
+    // ---------------- Cholesky (type T), no per-lane branches ---------------
+    // Step k: the lanes publish column k of their rows, read the pivot and the
+    // column, update their rows with the unscaled column (l_ik l_jk = a_ik a_jk /
+    // a_kk) and scale their column-k entry (the owner's becomes sqrt(a_kk)).
+    // Rows above k publish and update entries that are never read.
+    bool bad = false;
+    double pmin = 1e300;
+#pragma unroll
+    for (int k = 0; k < MR; ++k) {
+        if (k < S) col[i0] = L0[k < S ? k : 0];
+        col[i1] = L1[k];
+        __syncthreads();
+        const T dkk = col[k];
+        bad = bad || !(dkk > (T)0);
+        pmin = fmin(pmin, (double)dkk);
+        const T inv = (T)1 / sqrt(dkk);
+        if (t == 0) dinv[k] = inv;
+        if (k < S) {
+            const T tr = L0[k] * inv * inv;
+#pragma unroll
+            for (int j = k + 1; j < S; ++j) L0[j] = fma(-tr, col[j], L0[j]);
+            L0[k] = L0[k] * inv;
+        }
+        {
+            const T tr = L1[k] * inv * inv;
+#pragma unroll
+            for (int j = k + 1; j < MR; ++j) L1[j] = fma(-tr, col[j], L1[j]);
+            L1[k] = L1[k] * inv;
+        }
+        __syncthreads();
+    }
+    // keep the strictly lower part only (diagonal and above cleared)
+#pragma unroll
+    for (int j = 0; j < S; ++j) L0[j] = j < i0 ? L0[j] : (T)0;
+#pragma unroll
+    for (int j = 0; j < MR; ++j) L1[j] = j < i1 ? L1[j] : (T)0;
+    const T dv0 = dinv[i0], dv1 = dinv[i1];
+
+    // FP32 form: ill-conditioned or failed blocks are redone in FP64
+    const bool ill = REFINE && (bad || !(dmax / pmin <= kSegPivotRatioMax));
+    if (ill && pv && t == 0) {
+        const unsigned long long q = atomicAdd(ex.flag_count, 1ULL);
+        ex.flag_list[q] = p;
+    }
+    if (!REFINE && bad && pv && t == 0) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(&a.hdr->not_pd), 1ULL);
+        if (a.pt)
+            for (int q = 0; q < a.pt_stride; ++q) a.pt[p * a.pt_stride + q] = __builtin_nan("");
+        if (a.mean)
+            for (int o = 0; o < ny; ++o) a.mean[p * ny + o] = __builtin_nan("");
+        if (a.stdv) a.stdv[p] = __builtin_nan("");
+    }
+    const bool live = pv && !bad && !ill;
+
+    // L z = rhs for the lane's rows: with the strictly lower part in registers a
+    // row's running value stops changing once its own step has passed, and z_i
+    // is that value times 1 / L_ii.
+    auto fwd = [&](T& r0, T& r1) {
+#pragma unroll
+        for (int k = 0; k < MR; ++k) {
+            const int own = k < S ? k : MR - 1 - k;
+            const T zk = __shfl(k < S ? r0 : r1, own, S) * dinv[k];
+            if (k < S) r0 = fma(-L0[k < S ? k : 0], zk, r0);
+            r1 = fma(-L1[k], zk, r1);
+        }
+        r0 *= dv0;
+        r1 *= dv1;
+    };
+    // L^T x = z for the lane's rows (x starts at 0: rows not yet solved add
+    // nothing through the cleared entries)
+    auto bwd = [&](T& r0, T& r1) {
+        T x0 = (T)0, x1 = (T)0;
+#pragma unroll
+        for (int k = MR - 1; k >= 0; --k) {
+            T part = fma(L1[k], x1, k < S ? L0[k < S ? k : 0] * x0 : (T)0);
+#pragma unroll
+            for (int off = S / 2; off > 0; off >>= 1) part += __shfl_xor(part, off, S);
+            const int own = k < S ? k : MR - 1 - k;
+            const T xk = (__shfl(k < S ? r0 : r1, own, S) - part) * dinv[k];
+            if (k < S) {
+                if (t == k) x0 = xk;
+            } else {
+                if (t == own) x1 = xk;
+            }
+        }
+        r0 = x0;
+        r1 = x1;
+    };
+    // x = K^-1 rhs for the lane's rows; FP32 form: one FP64-residual refinement
+    auto solve = [&](double rhs0, double rhs1, double& x0, double& x1) {
+        T u0 = (T)rhs0, u1 = (T)rhs1;
+        fwd(u0, u1);
+        bwd(u0, u1);
+        x0 = (double)u0;
+        x1 = (double)u1;
+        if constexpr (REFINE) {
+            xv[i0] = x0;
+            xv[i1] = x1;
+            __syncthreads();
+            double q0 = rhs0, q1 = rhs1;
+#pragma unroll 1
+            for (int j = 0; j < MR; ++j) {
+                const double xj = xv[j];
+                const double K0 = j <= i0 ? Kt[tri(i0, j)] : Kt[tri(j, i0)];
+                const double K1 = j <= i1 ? Kt[tri(i1, j)] : Kt[tri(j, i1)];
+                q0 = fma(-K0, xj, q0);
+                q1 = fma(-K1, xj, q1);
+            }
+            __syncthreads();
+            T w0 = (T)q0, w1 = (T)q1;
+            fwd(w0, w1);
+            bwd(w0, w1);
+            x0 += (double)w0;
+            x1 += (double)w1;
+        }
+    };
+
+    // ---------------- sigma^2 ----------------------------------------------
+    const bool full = REFINE || (a.grad && a.ys);
+    double s2, bv0 = 0.0, bv1 = 0.0, zb0 = 0.0, zb1 = 0.0;
+    if (full) {
+        solve(k0, k1, bv0, bv1);
+        s2 = c - seg_sum<S>(k0 * bv0 + k1 * bv1);
+    } else {
+        T u0 = (T)k0, u1 = (T)k1;
+        fwd(u0, u1);
+        zb0 = (double)u0;
+        zb1 = (double)u1;
+        s2 = c - seg_sum<S>(zb0 * zb0 + zb1 * zb1);
+    }
+    const double sig = sqrt(s2);
+    if (live && a.stdv && t == 0) a.stdv[p] = sig;
+
+    // ---------------- outputs ----------------------------------------------
+    double ll = 0.0, Gs = 0.0, at0 = 0.0, at1 = 0.0;
+    for (int o = 0; o < ny; ++o) {
+        const double y0 = v0 ? a.y[(int64_t)j0 * ny + o] : 0.0;
+        const double y1 = v1 ? a.y[(int64_t)j1 * ny + o] : 0.0;
+        double mu, av0 = 0.0, av1 = 0.0;
+        if (full) {
+            solve(y0, y1, av0, av1);
+            mu = seg_sum<S>(k0 * av0 + k1 * av1);
+        } else {
+            T u0 = (T)y0, u1 = (T)y1;
+            fwd(u0, u1);
+            mu = seg_sum<S>(zb0 * (double)u0 + zb1 * (double)u1);
+        }
+        if (live && a.mean && t == 0) a.mean[p * ny + o] = mu;
+        if (a.ys) {
+            const double yo = pv ? a.ys[p * ny + o] : 0.0;
+            const double res = yo - mu;
+            ll += -0.5 * res * res / s2 - kHalfLog2PiS - log(sig);
+            if (a.grad) {
+                Gs += -0.5 / s2 + 0.5 * res * res / (s2 * s2);
+                at0 = fma(res / s2, av0, at0);
+                at1 = fma(res / s2, av1, at1);
+            }
+        }
+    }
+    if (live && a.pt && t == 0) a.pt[p * a.pt_stride] = ll;
+
+    // ---------------- adjoint contraction ----------------------------------
+    if (a.grad && a.pt) {
+        xv[i0] = bv0;
+        xv[i1] = bv1;
+        vbuf[i0] = at0;
+        vbuf[i1] = at1;
+        __syncthreads();
+        const bool ard = kp.n_ls > 1;
+        double acc_w = 0.0, acc_C = 0.0, acc_s = 0.0, acc_l = 0.0;
+        double accf[kSegMaxDim];
+#pragma unroll
+        for (int f = 0; f < kSegMaxDim; ++f) accf[f] = 0.0;
+        const double* xs = X + MR * d;
+        // one row i of this lane: its self term and its pairs (i, j <= i)
+        auto row_terms = [&](int i, bool vi, double bi, double ati, double eis) {
+            if (!vi) return;
+            const double* xi = X + i * d;
+            const double v = ati - 2.0 * Gs * bi;
+            acc_C += v;
+            acc_s += v * eis;
+            if (kp.n_ls == 1) {
+                double u = 0.0;
+                for (int f = 0; f < d; ++f) {
+                    const double tt = xi[f] - xs[f];
+                    u = fma(tt, tt, u);
+                }
+                acc_l += v * eis * 2.0 * u * kp.h[0];
+            } else if (ard) {
+#pragma unroll
+                for (int f = 0; f < kSegMaxDim; ++f) {
+                    if (f < d) {
+                        const double tt = xi[f] - xs[f];
+                        accf[f] += v * eis * 2.0 * tt * tt * kp.h[f];
+                    }
+                }
+            }
+#pragma unroll 1
+            for (int j = 0; j <= i; ++j) {
+                if (!row_valid<MR>(j, vlo, vhi)) continue;
+                const double bj = xv[j], aj = vbuf[j];
+                const double mult = j == i ? 1.0 : 2.0;
+                const double W = mult * (Gs * bi * bj - 0.5 * (bi * aj + ati * bj));
+                const double Ev = j == i ? 1.0 : e_of(Kt[tri(i, j)]);
+                if (j == i) acc_w += W;
+                acc_C += W;
+                acc_s += W * Ev;
+                const double* xj = X + j * d;
+                if (kp.n_ls == 1) {
+                    double u = 0.0;
+                    for (int f = 0; f < d; ++f) {
+                        const double tt = xi[f] - xj[f];
+                        u = fma(tt, tt, u);
+                    }
+                    acc_l += W * Ev * 2.0 * u * kp.h[0];
+                } else if (ard) {
+#pragma unroll
+                    for (int f = 0; f < kSegMaxDim; ++f) {
+                        if (f < d) {
+                            const double tt = xi[f] - xj[f];
+                            accf[f] += W * Ev * 2.0 * tt * tt * kp.h[f];
+                        }
+                    }
+                }
+            }
+        };
+        row_terms(i0, v0, bv0, at0, e0s);
+        row_terms(i1, v1, bv1, at1, e1s);
+        acc_w = seg_sum<S>(acc_w);
+        acc_C = seg_sum<S>(acc_C);
+        acc_s = seg_sum<S>(acc_s);
+        acc_l = seg_sum<S>(acc_l);
+        double* gp = a.pt + p * a.pt_stride + 1;
+        if (live && t == 0) {
+            for (int q = 0; q < a.n_theta; ++q) gp[q] = 0.0;
+            if (kp.t_w >= 0) gp[kp.t_w] += kp.w * (acc_w + Gs);
+            if (kp.t_C >= 0) gp[kp.t_C] += kp.C * (acc_C + Gs);
+            if (kp.t_s >= 0) gp[kp.t_s] += kp.s * (acc_s + Gs);
+            if (kp.t_l >= 0 && kp.n_ls == 1) gp[kp.t_l] += kp.s * acc_l;
+        }
+        if (kp.t_l >= 0 && ard) {
+#pragma unroll
+            for (int f = 0; f < kSegMaxDim; ++f) {
+                if (f < d) {
+                    const double af = seg_sum<S>(accf[f]);
+                    if (live && t == 0) gp[kp.t_l + f] += kp.s * af;
+                }
+            }
+        }
+    }
+}
+
+template <typename T, int MR>
+size_t seg_lds(int d) {
+    constexpr int S = MR / 2;
+    constexpr int G = 64 / S;
+    constexpr int TRI = MR * (MR + 1) / 2;
+    const size_t doubles = (size_t)G * ((size_t)(MR + 1) * d + TRI + 2 * MR);
+    return doubles * sizeof(double) + (size_t)G * 2 * MR * sizeof(T);
+}
+
+template <typename T, int MR>
+void launch_seg_mr(const VecArgs& a, const RegExtra& ex, int64_t max_points, hipStream_t st) {
+    constexpr int G = 64 / (MR / 2);
+    int64_t blocks = (max_points + G - 1) / G;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL((vecchia_seg_kernel<T, MR>), dim3((unsigned)blocks), dim3(64),
+                       (seg_lds<T, MR>(a.d)), st, a, ex);
+}
+
+}  // namespace
+
+bool seg_supported(int m, int d) { return m >= 1 && m <= 32 && d >= 1 && d <= kSegMaxDim; }
+
+// FP32 factor + FP64 refinement form (flagged blocks are queued in ex.flag_list).
+int launch_seg(const VecArgs& a, const RegExtra& ex, int64_t max_points, hipStream_t st) {
+    if (!seg_supported(a.m, a.d)) return -1;
+    if (a.m <= 16) launch_seg_mr<float, 16>(a, ex, max_points, st);
+    else launch_seg_mr<float, 32>(a, ex, max_points, st);
+    return 0;
+}
+
+}  // namespace nngp
