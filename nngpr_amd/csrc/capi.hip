@@ -160,20 +160,22 @@ int run_fused(const VecArgs& a, const VecLayout& L, int precision, hipStream_t s
         hipLaunchKernelGGL(zero_flag_kernel, dim3(1), dim3(64), 0, st, L.flag_count);
         ex.flag_list = L.flag_list;
         ex.flag_count = L.flag_count;
+        const bool seg = seg_supported(a.m, a.d);
         timing_begin(tk, st);
-        const int rc = seg_supported(a.m, a.d) ? launch_seg(a, ex, a.npts, st)
-                                               : launch_reg(a, ex, true, a.npts, st);
+        const int rc = seg ? launch_seg(a, ex, true, a.npts, st) : launch_reg(a, ex, true, a.npts, st);
         timing_end(tk, st);
         if (rc) return NNGP_ERR_UNSUPPORTED;
         RegExtra redo{};
         redo.plist = L.flag_list;
         redo.pcount = L.flag_count;
-        const int64_t cap = a.npts < 4096 ? a.npts : 4096;
-        if (launch_reg(a, redo, false, cap, st)) return NNGP_ERR_UNSUPPORTED;
+        // the flagged points (a device-side count): a grid sized for the bound
+        if ((seg ? launch_seg(a, redo, false, a.npts, st) : launch_reg(a, redo, false, a.npts < 4096 ? a.npts : 4096, st)))
+            return NNGP_ERR_UNSUPPORTED;
         hipLaunchKernelGGL(add_refactored_kernel, dim3(1), dim3(64), 0, st, L.flag_count, L.hdr);
     } else {
         timing_begin(tk, st);
-        const int rc = launch_reg(a, ex, false, a.npts, st);
+        const int rc = seg_supported(a.m, a.d) ? launch_seg(a, ex, false, a.npts, st)
+                                               : launch_reg(a, ex, false, a.npts, st);
         timing_end(tk, st);
         if (rc) return NNGP_ERR_UNSUPPORTED;
     }

@@ -15,9 +15,11 @@ int reg_mr(int m);
 size_t reg_lds_bytes(int m, int d, bool fp32);
 int launch_reg(const VecArgs& a, const RegExtra& ex, bool fp32, int64_t max_points, hipStream_t st);
 
-// Segment kernel of the FP32 + FP64-refinement form (m <= 32, d <= 8): 16 lanes
-// per block of 32 rows, factor in LDS.  Flagged blocks go to ex.flag_list.
+// Segment kernel (m <= 32, d <= 8): MR / 2 lanes per block of MR rows, two rows
+// per lane, factor in registers.  fp32: FP32 factor + FP64 refinement, flagged
+// blocks go to ex.flag_list; else the FP64 form.
 bool seg_supported(int m, int d);
-int launch_seg(const VecArgs& a, const RegExtra& ex, int64_t max_points, hipStream_t st);
+int launch_seg(const VecArgs& a, const RegExtra& ex, bool fp32, int64_t max_points,
+               hipStream_t st);
 
 }  // namespace nngp

@@ -62,8 +62,10 @@ __device__ __forceinline__ int opaque(int v) {
     return v;
 }
 
+// One group of G points starting at slot `base`.
 template <typename T, int MR>
-__global__ __launch_bounds__(64) void vecchia_seg_kernel(VecArgs a, RegExtra ex) {
+__device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, int64_t base,
+                                         int64_t count) {
     constexpr int S = MR / 2;
     constexpr int G = 64 / S;
     constexpr int TRI = MR * (MR + 1) / 2;
@@ -88,10 +90,6 @@ __global__ __launch_bounds__(64) void vecchia_seg_kernel(VecArgs a, RegExtra ex)
              seg * 2 * MR;
     T* dinv = col + MR;
 
-    // one group of G points per block
-    const int64_t count = ex.pcount ? (int64_t)*ex.pcount : a.npts;
-    const int64_t base = (int64_t)blockIdx.x * G;
-    if (base >= count) return;
     const int64_t slot = base + seg;
     const bool pv = slot < count;
     const int64_t p = pv ? (ex.plist ? ex.plist[slot] : slot) : 0;
@@ -439,6 +437,28 @@ __global__ __launch_bounds__(64) void vecchia_seg_kernel(VecArgs a, RegExtra ex)
     }
 }
 
+// One group of G points per block (no persistent loop: nothing loop-invariant is
+// kept live across groups).
+template <typename T, int MR>
+__global__ __launch_bounds__(64) void vecchia_seg_kernel(VecArgs a, RegExtra ex) {
+    constexpr int G = 64 / (MR / 2);
+    const int64_t count = ex.pcount ? (int64_t)*ex.pcount : a.npts;
+    const int64_t base = (int64_t)blockIdx.x * G;
+    if (base < count) seg_body<T, MR>(a, ex, base, count);
+}
+
+// Grid-stride form for point lists whose length is known on the device only.
+template <typename T, int MR>
+__global__ __launch_bounds__(64) void vecchia_seg_list_kernel(VecArgs a, RegExtra ex) {
+    constexpr int G = 64 / (MR / 2);
+    const int64_t count = (int64_t)*ex.pcount;
+#pragma unroll 1
+    for (int64_t base = (int64_t)blockIdx.x * G; base < count; base += (int64_t)gridDim.x * G) {
+        seg_body<T, MR>(a, ex, base, count);
+        __syncthreads();
+    }
+}
+
 template <typename T, int MR>
 size_t seg_lds(int d) {
     constexpr int S = MR / 2;
@@ -451,6 +471,14 @@ size_t seg_lds(int d) {
 template <typename T, int MR>
 void launch_seg_mr(const VecArgs& a, const RegExtra& ex, int64_t max_points, hipStream_t st) {
     constexpr int G = 64 / (MR / 2);
+    if (ex.pcount) {
+        // a device-side list: at most 1024 blocks striding over it
+        int64_t blocks = (max_points + G - 1) / G;
+        blocks = blocks < 1 ? 1 : (blocks > 1024 ? 1024 : blocks);
+        hipLaunchKernelGGL((vecchia_seg_list_kernel<T, MR>), dim3((unsigned)blocks), dim3(64),
+                           (seg_lds<T, MR>(a.d)), st, a, ex);
+        return;
+    }
     int64_t blocks = (max_points + G - 1) / G;
     if (blocks < 1) blocks = 1;
     hipLaunchKernelGGL((vecchia_seg_kernel<T, MR>), dim3((unsigned)blocks), dim3(64),
@@ -461,11 +489,18 @@ void launch_seg_mr(const VecArgs& a, const RegExtra& ex, int64_t max_points, hip
 
 bool seg_supported(int m, int d) { return m >= 1 && m <= 32 && d >= 1 && d <= kSegMaxDim; }
 
-// FP32 factor + FP64 refinement form (flagged blocks are queued in ex.flag_list).
-int launch_seg(const VecArgs& a, const RegExtra& ex, int64_t max_points, hipStream_t st) {
+// fp32: FP32 factor + FP64 refinement form (flagged blocks are queued in
+// ex.flag_list); else the FP64 form.
+int launch_seg(const VecArgs& a, const RegExtra& ex, bool fp32, int64_t max_points,
+               hipStream_t st) {
     if (!seg_supported(a.m, a.d)) return -1;
-    if (a.m <= 16) launch_seg_mr<float, 16>(a, ex, max_points, st);
-    else launch_seg_mr<float, 32>(a, ex, max_points, st);
+    if (fp32) {
+        if (a.m <= 16) launch_seg_mr<float, 16>(a, ex, max_points, st);
+        else launch_seg_mr<float, 32>(a, ex, max_points, st);
+    } else {
+        if (a.m <= 16) launch_seg_mr<double, 16>(a, ex, max_points, st);
+        else launch_seg_mr<double, 32>(a, ex, max_points, st);
+    }
     return 0;
 }
 
