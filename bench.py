@@ -7,10 +7,12 @@ A step is one log-likelihood + gradient evaluation over all training rows, as
 the reference computes it on every optimizer call (base_nngpr.py:530-614): the
 Vecchia kNN (each row's 32 nearest earlier rows) followed by the fused
 fill -> Cholesky -> mu/sigma -> loglik -> adjoint gradient pass.  `value` is the
-FP32 + FP64-refinement form (allow_downcast_f32='yes').  The line also reports
-the plain FP64 form, kNN-cached evaluation rates, predictions/s on the test
-set, the FP32-vs-FP64 errors, the dominant kernel's roofline position and the
-restated NumpyNngpr CPU path timed on this host.
+reference's default precision mode (allow_downcast_f32='only-nn': kNN screened
+in FP32 with exact FP64 keys, FP64 factor).  The line also reports the 'yes'
+(FP32 factor + FP64 refinement) and 'no' (FP64 everywhere) modes, kNN-cached
+evaluation rates, predictions/s on the test set, the FP32-vs-FP64 errors, the
+fused kernel's roofline position and the restated NumpyNngpr CPU path timed on
+this host.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -38,7 +40,41 @@ def parse():
     ap.add_argument("--n-test", type=int, default=43_487)
     ap.add_argument("--skip-extras", action="store_true",
                     help="only the headline timed region (used for profiling runs)")
+    ap.add_argument("--copy-probe-bytes", type=int, default=0,
+                    help="run only a plain device-to-device copy of this many bytes, 5 times "
+                         "(calibrates the HBM byte counters against a known traffic)")
     return ap.parse_args()
+
+
+TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                               "loglik_road_mixture_hbm_traffic.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (calibrated against a plain copy); None when absent."""
+    try:
+        text = open(TRAFFIC_PROFILE).read()
+    except OSError:
+        return None
+    for line in text.splitlines():
+        parts = line.split()
+        if len(parts) == 4 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+            return int(parts[3])
+    return None
+
+
+def copy_probe(nbytes):
+    """Five launches of torch's copy kernel over nbytes (reads nbytes, writes nbytes)."""
+    import torch
+    n = nbytes // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 4, "launches": 5}))
 
 
 def cpu_baseline(ds):
@@ -109,6 +145,9 @@ def serve_sweep(ds, rates, seconds=3.0, max_requests=300):
 
 def main():
     args = parse()
+    if args.copy_probe_bytes:
+        copy_probe(args.copy_probe_bytes)
+        return
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
@@ -139,17 +178,21 @@ def main():
     n, d = x.shape
     F32, F64 = hip_ops.PRECISION_FP32_REFINED, hip_ops.PRECISION_FP64
 
-    def evaluate(prec, nn=None):
-        knn_prec = F32 if prec == F32 else F64
+    # Precision modes of the reference (allow_downcast_f32): kNN form, factor form.
+    MODES = {"only-nn": (F32, F64), "yes": (F32, F32), "no": (F64, F64)}
+
+    def evaluate(mode, nn=None):
+        knn_prec, prec = MODES[mode]
         if nn is None:
             nn = hip_ops.knn(x, x, ls, m, ref_cut=0, precision=knn_prec)
         ll, g, _ = hip_ops.vecchia_loglik(x, y, nn, ks, alpha, precision=prec,
                                           eval_gradient=True, check_pd=False)
         return ll, g, nn
 
-    def predict(prec, nn=None):
+    def predict(mode, nn=None):
+        knn_prec, prec = MODES[mode]
         if nn is None:
-            nn = hip_ops.knn(x, xq, ls, m, precision=F32 if prec == F32 else F64)
+            nn = hip_ops.knn(x, xq, ls, m, precision=knn_prec)
         return hip_ops.predict(x, y, xq, nn, ks, alpha, precision=prec, check_pd=False)
 
     def timed(fn, steps, warmup):
@@ -162,10 +205,6 @@ def main():
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / steps
 
-    # ---------------- headline: FP32 + refinement, kNN recomputed per step ----
-    for _ in range(args.warmup):
-        evaluate(F32)
-    torch.cuda.synchronize()
     TIMED = {"knn_prefix_scan": nat.TIMED_KNN_SCAN, "knn_rerank": nat.TIMED_KNN_RERANK,
              "knn_grid_build": nat.TIMED_KNN_GRID_BUILD, "knn_grid_search": nat.TIMED_KNN_GRID,
              "fused_loglik_grad": nat.TIMED_LOGLIK}
@@ -190,30 +229,54 @@ def main():
             out[name + "_launches"] = cnt / steps
         return wall, out
 
-    step_s, stages = stage_times(lambda: evaluate(F32), args.steps)
+    def stage_ms(st):
+        return {k: v for k, v in st.items() if not k.endswith("_launches")}
+
+    # ---------------- headline: the reference's default mode ('only-nn') -------
+    HEAD = "only-nn"
+    for _ in range(args.warmup):
+        evaluate(HEAD)
+    torch.cuda.synchronize()
+    step_s, stages = stage_times(lambda: evaluate(HEAD), args.steps)
     fused_avg = stages["fused_loglik_grad"] / max(stages["fused_loglik_grad_launches"], 1)
 
-    # Dominant kernel of the step: the fused fill -> Cholesky -> solve -> loglik ->
-    # adjoint pass.  Algorithmic flop per point (m neighbours, d features,
+    # Roofline of the fused fill -> Cholesky -> solve -> loglik -> adjoint pass
+    # (FP64 VALU).  Algorithmic flop per point (m neighbours, d features,
     # E = (m+1)(m+2)/2 covariance entries): Cholesky 2m^3/3, two triangular solves
     # 4m^2, fill E(3d+2), adjoint E(d+3) + 2m^2.
+    full_size = (n, int(xq.shape[0])) == (434_874, 43_487)
     E = (m + 1) * (m + 2) / 2.0
     fused_flop_pt = 2.0 * m ** 3 / 3.0 + 4.0 * m * m + E * (3 * d + 2) + E * (d + 3) + 2.0 * m * m
     fused_flop = fused_flop_pt * n
     achieved = fused_flop / (fused_avg * 1e-3) / 1e12
-    roofline = {"kernel": "vecchia_reg_kernel<float,32> fused loglik+grad (FP32 factor, FP64 fill "
-                          "and refinement)", "bound": "valu",
-                "achieved": achieved, "peak": PEAK_FP32_VALU_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP32_VALU_TFLOPS, "traffic": None,
+    roofline = {"kernel": "vecchia_seg_kernel<double,32> fused loglik+grad (FP64)", "bound": "valu",
+                "achieved": achieved, "peak": PEAK_FP64_VALU_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP64_VALU_TFLOPS,
+                # the committed counter summary covers the default workload only
+                "traffic": measured_traffic("vecchia_seg_kernel<double,32>") if full_size else None,
+                "traffic_source": "profiles/loglik_road_mixture_hbm_traffic.md" if full_size else None,
                 "avg_ms": fused_avg, "algorithmic_flop": fused_flop,
                 "algorithmic_flop_per_point": fused_flop_pt}
 
-    extras = {}
+    extras = {"stage_ms": stage_ms(stages)}
     if not args.skip_extras:
-        # the same step with the brute-force kNN scans (FP32 screen + FP64 re-rank)
+        nq = xq.shape[0]
+        res = {}
+        for mode in MODES:
+            t_eval = step_s if mode == HEAD else timed(lambda: evaluate(mode), args.steps, 1)
+            ll, g, nn = evaluate(mode)
+            t_cached = timed(lambda: evaluate(mode, nn), args.steps, 1)
+            t_pred = timed(lambda: predict(mode), args.steps, 1)
+            mean, std = predict(mode)
+            res[mode] = dict(t_eval=t_eval, t_cached=t_cached, t_pred=t_pred, ll=ll.item(),
+                             g=g.cpu().numpy(), nn=nn, mean=mean.cpu().numpy()[:, 0],
+                             std=std.cpu().numpy())
+        _, stages32 = stage_times(lambda: evaluate("yes"), max(2, args.steps // 2))
+        # the same headline step with the brute-force kNN scans (FP32 screen +
+        # FP64 re-rank) instead of the grid search
         with hip_ops.knn_method(hip_ops.KNN_BRUTE_FORCE):
-            evaluate(F32)
-            tbf, stages_bf = stage_times(lambda: evaluate(F32), max(2, args.steps // 2))
+            evaluate(HEAD)
+            tbf, stages_bf = stage_times(lambda: evaluate(HEAD), max(2, args.steps // 2))
         scan_ms = stages_bf["knn_prefix_scan"]
         pairs = n * (n - 1) / 2.0
         knn_flops = 3.0 * d * pairs  # sub, mul, add per feature per pair
@@ -222,61 +285,41 @@ def main():
                          "bound": "valu", "achieved": ach_scan, "peak": PEAK_FP32_VALU_TFLOPS,
                          "unit": "TFLOP/s", "frac": ach_scan / PEAK_FP32_VALU_TFLOPS,
                          "avg_ms": scan_ms, "algorithmic_flop": knn_flops}
-        t64 = timed(lambda: evaluate(F64), max(2, args.steps // 2), 1)
-        ll64, g64, nn64 = evaluate(F64)
-        ll32, g32, nn32 = evaluate(F32)
-        knn_stats = hip_ops.knn_stats(dev=dev)
-        knn_exact = bool(torch.equal(nn64, nn32))
-        t32c = timed(lambda: evaluate(F32, nn32), args.steps, 1)
-        t64c = timed(lambda: evaluate(F64, nn64), args.steps, 1)
-        nq = xq.shape[0]
-        tp32 = timed(lambda: predict(F32), args.steps, 1)
-        tp64 = timed(lambda: predict(F64), args.steps, 1)
-        mean64, std64 = predict(F64)
-        mean32, std32 = predict(F32)
-        m64, s64 = mean64.cpu().numpy()[:, 0], std64.cpu().numpy()
-        m32, s32 = mean32.cpu().numpy()[:, 0], std32.cpu().numpy()
-        g64n, g32n = g64.cpu().numpy(), g32.cpu().numpy()
-        rel_m = np.abs(m32 - m64) / np.maximum(np.abs(m64), 1e-300)
-        rel_s = np.abs(s32 - s64) / np.abs(s64)
-        nat.timing_enable(True)
-        nat.timing_read(nat.TIMED_LOGLIK)
-        evaluate(F64, nn64)
-        torch.cuda.synchronize()
-        f64_ms, _ = nat.timing_read(nat.TIMED_LOGLIK)
-        nat.timing_enable(False)
-        extras = {
-            "evals_per_s_fp32_refined": 1.0 / step_s,
-            "evals_per_s_fp64": 1.0 / t64,
-            "evals_per_s_fp32_refined_knn_cached": 1.0 / t32c,
-            "evals_per_s_fp64_knn_cached": 1.0 / t64c,
-            "preds_per_s_fp32_refined": nq / tp32,
-            "preds_per_s_fp64": nq / tp64,
-            "speedup_fp32_vs_fp64": {"evals": t64 / step_s, "evals_knn_cached": t64c / t32c,
-                                     "preds": tp64 / tp32},
+        r32, r64 = res["yes"], res["no"]
+        rel_m = np.abs(r32["mean"] - r64["mean"]) / np.maximum(np.abs(r64["mean"]), 1e-300)
+        rel_s = np.abs(r32["std"] - r64["std"]) / np.abs(r64["std"])
+        extras.update({
+            "evals_per_s": {k: 1.0 / v["t_eval"] for k, v in res.items()},
+            "evals_per_s_knn_cached": {k: 1.0 / v["t_cached"] for k, v in res.items()},
+            "preds_per_s": {k: nq / v["t_pred"] for k, v in res.items()},
+            "speedup_fp32_vs_fp64": {
+                "evals": r64["t_eval"] / r32["t_eval"],
+                "evals_knn_cached": r64["t_cached"] / r32["t_cached"],
+                "preds": r64["t_pred"] / r32["t_pred"]},
             "fp32_vs_fp64_error": {
-                "abs_dloglik_over_n": abs(ll32.item() - ll64.item()) / n,
-                "grad_max_rel": float(np.max(np.abs(g32n - g64n) / np.abs(g64n))),
+                "abs_dloglik_over_n": abs(r32["ll"] - r64["ll"]) / n,
+                "grad_max_rel": float(np.max(np.abs(r32["g"] - r64["g"]) / np.abs(r64["g"]))),
                 "mean_rel_max": float(rel_m.max()), "mean_rel_mean": float(rel_m.mean()),
                 "std_rel_max": float(rel_s.max()), "std_rel_mean": float(rel_s.mean()),
             },
-            "knn_indices_identical_fp32_vs_fp64": knn_exact,
-            "knn_rows_recomputed_fp32": knn_stats["knn_rows_recomputed"] if knn_stats else None,
-            "stage_ms_fp32": {k: v for k, v in stages.items() if not k.endswith("_launches")},
-            "stage_ms_fp64": {"fused_loglik_grad": f64_ms},
-            "evals_per_s_fp32_refined_brute_force_knn": 1.0 / tbf,
-            "stage_ms_fp32_brute_force_knn": {k: v for k, v in stages_bf.items()
-                                              if not k.endswith("_launches")},
+            "default_vs_fp64_loglik_identical": res[HEAD]["ll"] == r64["ll"],
+            "knn_indices_identical_across_modes": bool(
+                torch.equal(res["yes"]["nn"], res["no"]["nn"]) and
+                torch.equal(res[HEAD]["nn"], res["no"]["nn"])),
+            "stage_ms_fp32_form": stage_ms(stages32),
+            "evals_per_s_brute_force_knn": 1.0 / tbf,
+            "stage_ms_brute_force_knn": stage_ms(stages_bf),
             "roofline_knn_brute_force_scan": roofline_scan,
-            "loglik_fp64": ll64.item(),
-        }
+            "loglik": r64["ll"],
+        })
 
     serving = None
     if not args.skip_extras:
         serving = serve_sweep(ds, (50, 200, 1000))
 
     line = {
-        "metric": "loglik+grad evals/s (FP32+FP64 refinement form, kNN recomputed per eval)",
+        "metric": "loglik+grad evals/s (reference default precision 'only-nn': FP64 factor, "
+                  "kNN recomputed per eval)",
         "value": 1.0 / step_s,
         "unit": "evals/s",
         "n_gpus": 1,
@@ -286,7 +329,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32+fp64",
+        "dtype": "fp64",
         "data": "synthetic",
         "config": {"workload": "road_mixture: n_train=%d n_test=%d d=3 m=32, "
                                "White(0.01)+C(1.0)*RBF(0.1)" % (n, xq.shape[0]),
