@@ -50,6 +50,68 @@ __device__ __forceinline__ void topk_insert(T (&keys)[K], int (&ids)[K], T xk, i
     }
 }
 
+// Sorting networks over (key, index) pairs in registers, for lists whose keys are
+// never NaN (-inf / +inf allowed): keys move with fmin / fmax (equal keys are
+// interchangeable), indices follow the pair order (indices unsigned, as above).
+__device__ __forceinline__ bool pair_less(double a, int ai, double b, int bi) {
+    return (a < b) | ((a == b) & ((unsigned)ai < (unsigned)bi));
+}
+
+// Ascending compare-exchange: afterwards (ka, ia) <= (kb, ib).
+__device__ __forceinline__ void pair_cex(double& ka, int& ia, double& kb, int& ib) {
+    const bool sw = pair_less(kb, ib, ka, ia);
+    const double lo = fmin(ka, kb), hi = fmax(ka, kb);
+    const int il = sw ? ib : ia, ih = sw ? ia : ib;
+    ka = lo;
+    kb = hi;
+    ia = il;
+    ib = ih;
+}
+
+// Bitonic sort of N (a power of two) pairs, ascending.
+template <int N>
+__device__ __forceinline__ void pair_sort(double (&k)[N], int (&id)[N]) {
+#pragma unroll
+    for (int size = 2; size <= N; size <<= 1) {
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                const int j = i ^ stride;
+                if (j > i) {
+                    if ((i & size) == 0) pair_cex(k[i], id[i], k[j], id[j]);
+                    else pair_cex(k[j], id[j], k[i], id[i]);
+                }
+            }
+        }
+    }
+}
+
+// Merges B pairs (any order) into a sorted list of K pairs, keeping the K
+// smallest: sort the B pairs, take C_t = min(A_t, B_{K-1-t}) over the last B
+// slots (with the sorted B padded by +inf this is min(A_t, B'_{K-1-t}) over all
+// t, a bitonic sequence holding the K smallest of both), then half-cleaners.
+template <int K, int B>
+__device__ __forceinline__ void pair_merge(double (&keys)[K], int (&ids)[K], double (&bk)[B],
+                                           int (&bi)[B]) {
+    pair_sort<B>(bk, bi);
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+        const int t = K - B + i, j = B - 1 - i;
+        const bool lt = pair_less(bk[j], bi[j], keys[t], ids[t]);
+        keys[t] = fmin(bk[j], keys[t]);
+        ids[t] = lt ? bi[j] : ids[t];
+    }
+#pragma unroll
+    for (int stride = K >> 1; stride > 0; stride >>= 1) {
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            const int j = i ^ stride;
+            if (j > i) pair_cex(keys[i], ids[i], keys[j], ids[j]);
+        }
+    }
+}
+
 // FP64 key of a (ref, query) pair: fma(t_{d-1}, t_{d-1}, ... fma(t_1, t_1, t_0 * t_0)),
 // t_f = u_ref,f - u_query,f.
 __device__ __forceinline__ double exact_key(const double* ur, const double* uq, int d) {

@@ -365,6 +365,25 @@ __device__ __forceinline__ void store_list(const int (&ids)[K], int pad, int32_t
         if (t >= pad) o[t - pad] = ids[t];
 }
 
+// Merges a lane's buffered candidates in slots [off, min(off + S, cnt)) (LDS
+// column `lane`) into its sorted list.
+template <int K, int S>
+__device__ __forceinline__ void merge_slots(double (&keys)[K], int (&ids)[K],
+                                            const double (*bk)[64], const int (*bi)[64], int lane,
+                                            int off, int cnt) {
+    double ck[S];
+    int ci[S];
+#pragma unroll
+    for (int b = 0; b < S; ++b) {
+        const bool v = off + b < cnt;
+        const double x = bk[off + b][lane];
+        const int xi = bi[off + b][lane];
+        ck[b] = v ? x : __builtin_inf();
+        ci[b] = v ? xi : -1;
+    }
+    pair_merge<K, S>(keys, ids, ck, ci);
+}
+
 // One lane per query, 64 neighbouring queries per wave.  A lane walks the rings
 // of cells around its own cell (R = 0, 1, ...; a ring's rows (y, z) one after the
 // other: a new row's whole x run, an old row's two new end cells) and appends the
@@ -459,12 +478,24 @@ void lane_knn_kernel(SearchArgs a) {
         for (;;) {
             if (__builtin_amdgcn_ballot_w64((cnt > B - CH) | (ring_end & (cnt > 0)))) {
                 const int nb = wave_max32(cnt);
+                // K = 16: insertion ladders (the networks would not fit its 3-wave
+                // register budget); K = 32: one network over all slots; K = 64:
+                // two halves (limits its spills)
+                if (K == 16) {
 #pragma unroll 1
-                for (int b = 0; b < nb; ++b) {
-                    const bool v = b < cnt;
-                    const double x = bk[b][lane];
-                    const int xi = bi[b][lane];
-                    topk_insert<double, K>(keys, ids, v ? x : __builtin_inf(), v ? xi : -1);
+                    for (int b = 0; b < nb; ++b) {
+                        const bool v = b < cnt;
+                        const double x = bk[b][lane];
+                        const int xi = bi[b][lane];
+                        topk_insert<double, K>(keys, ids, v ? x : __builtin_inf(), v ? xi : -1);
+                    }
+                } else if (nb <= B / 2) {
+                    merge_slots<K, B / 2>(keys, ids, bk, bi, lane, 0, cnt);
+                } else if (K == 32) {
+                    merge_slots<K, (K == 32 ? B : B / 2)>(keys, ids, bk, bi, lane, 0, cnt);
+                } else {
+                    merge_slots<K, B / 2>(keys, ids, bk, bi, lane, 0, cnt);
+                    merge_slots<K, B / 2>(keys, ids, bk, bi, lane, B / 2, cnt);
                 }
                 cnt = 0;
                 if (active) {
