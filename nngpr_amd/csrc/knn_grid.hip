@@ -456,6 +456,9 @@ void lane_knn_kernel(SearchArgs a) {
         // admission threshold pair; inactive lanes admit nothing
         double tk = active ? __builtin_inf() : -__builtin_inf();
         int ti = -1;
+        // squared search radius with slack, (sqrt(tk) + tol)^2; +inf until k refs
+        double rr = __builtin_inf();
+        const double inv_h = P.inv_h;
         int cnt = 0;
         // per-lane walk: ring R, segment sg of the ring (two per row), refs [p, e)
         int R = 0, sg = 0, nseg = 0, ny = 1;
@@ -501,6 +504,10 @@ void lane_knn_kernel(SearchArgs a) {
                 if (active) {
                     tk = keys[K - 1];
                     ti = ids[K - 1];
+                    if (tk < __builtin_inf()) {
+                        const double r = sqrt(tk) + tol;
+                        rr = r * r;
+                    }
                 }
             }
             if (ring_end) {
@@ -568,6 +575,35 @@ void lane_knn_kernel(SearchArgs a) {
                 } else {
                     const int xc = part == 0 ? c[0] - R : c[0] + R;
                     if (xc >= 0 && xc < g[0]) cx0 = cx1 = xc;
+                }
+                // Clip to the ball of squared radius rr around the query: a ref
+                // outside it has key > tk (rr includes the slack tol), so it would
+                // not be admitted.  Cells [x_lo, x_hi] hold every ref whose x lies
+                // in [q_x - rx, q_x + rx] (same floor map as the binning).
+                if ((cx0 <= cx1) & (rr < __builtin_inf())) {
+                    double s2 = 0.0;
+                    if (D >= 2) {
+                        const double ylo = lo[1] + (double)y * h;
+                        const double dy = fmax(fmax(ylo - qu[1], qu[1] - (ylo + h)), 0.0);
+                        s2 = dy * dy;
+                    }
+                    if (D >= 3) {
+                        const double zlo = lo[2] + (double)z * h;
+                        const double dz = fmax(fmax(zlo - qu[2], qu[2] - (zlo + h)), 0.0);
+                        s2 = fma(dz, dz, s2);
+                    }
+                    const double rem = rr - s2;
+                    if (rem < 0.0) {
+                        cx0 = 1;
+                        cx1 = 0;
+                    } else {
+                        const double rx = sqrt(rem);
+                        const double gx = (double)g[0];
+                        const double fl = fmin(fmax(floor((qu[0] - rx - lo[0]) * inv_h), -1.0), gx);
+                        const double fh = fmin(fmax(floor((qu[0] + rx - lo[0]) * inv_h), -1.0), gx);
+                        cx0 = max(cx0, (int)fl);
+                        cx1 = min(cx1, (int)fh);
+                    }
                 }
                 if (cx0 <= cx1) {
                     p = cs[base + cx0];
