@@ -62,13 +62,34 @@ __device__ __forceinline__ int opaque(int v) {
     return v;
 }
 
-// One group of G points starting at slot `base`.
-template <typename T, int MR>
+// Entry e of a lane (0 <= e < MR + 3): (i0, 0..i0), (i1, 0..i1), then (i0, MR) and
+// (i1, MR), MR standing for the point itself.  `first`: the entry is on row i0.
+struct SegEntry {
+    int i, j;
+    bool first;
+};
+template <int MR>
+__device__ __forceinline__ SegEntry seg_entry(int e, int i0, int i1) {
+    const bool in0 = e <= i0;
+    const bool in1 = !in0 & (e <= MR);
+    const bool self0 = e == MR + 1;
+    SegEntry s;
+    s.first = in0 | self0;
+    s.i = s.first ? i0 : i1;
+    s.j = in0 ? e : (in1 ? e - i0 - 1 : MR);
+    return s;
+}
+
+// One group of G points starting at slot `base`.  Coordinates are held with DM
+// features (DM >= d, zero padded).
+template <typename T, int MR, int DM>
 __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, int64_t base,
                                          int64_t count) {
     constexpr int S = MR / 2;
     constexpr int G = 64 / S;
     constexpr int TRI = MR * (MR + 1) / 2;
+    constexpr int NE = MR + 3;  // entries per lane
+    constexpr int U = 5;        // entries per step of the fill and adjoint loops
     constexpr bool REFINE = sizeof(T) == 4;
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x;
@@ -81,12 +102,15 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
     const KernParams& kp = a.kp;
 
     // LDS per block (G blocks): FP64 covariance (lower triangle), two FP64
-    // vectors, X; then the factor's column broadcast and inverse diagonal (T).
+    // vectors, X, the point's own target; then the factor's column broadcast and
+    // inverse diagonal (T).
     double* Kt = reinterpret_cast<double*>(smem) + seg * TRI;
     double* xv = reinterpret_cast<double*>(smem) + G * TRI + seg * MR;
     double* vbuf = reinterpret_cast<double*>(smem) + G * (TRI + MR) + seg * MR;
-    double* X = reinterpret_cast<double*>(smem) + G * (TRI + 2 * MR) + seg * (MR + 1) * d;
-    T* col = reinterpret_cast<T*>(reinterpret_cast<double*>(smem) + G * (TRI + 2 * MR + (MR + 1) * d)) +
+    double* X = reinterpret_cast<double*>(smem) + G * (TRI + 2 * MR) + seg * (MR + 1) * DM;
+    double* yself = reinterpret_cast<double*>(smem) + G * (TRI + 2 * MR + (MR + 1) * DM) + seg;
+    T* col = reinterpret_cast<T*>(reinterpret_cast<double*>(smem) +
+                                  G * (TRI + 2 * MR + (MR + 1) * DM + 1)) +
              seg * 2 * MR;
     T* dinv = col + MR;
 
@@ -95,73 +119,93 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
     const int64_t p = pv ? (ex.plist ? ex.plist[slot] : slot) : 0;
 
     // ---------------- gather ---------------------------------------------
+    // Coordinates of the lane's rows (registers and LDS), the point's own
+    // coordinates, and the first target column (vbuf: neighbours, yself: own).
     const int j0 = (pv && i0 < a.m) ? a.nn[p * a.m + i0] : -1;
     const int j1 = (pv && i1 < a.m) ? a.nn[p * a.m + i1] : -1;
     const bool v0 = j0 >= 0, v1 = j1 >= 0;
-    for (int f = 0; f < d; ++f) {
-        X[i0 * d + f] = v0 ? a.x[(int64_t)j0 * d + f] : 0.0;
-        X[i1 * d + f] = v1 ? a.x[(int64_t)j1 * d + f] : 0.0;
+    double xr0[DM], xr1[DM];
+#pragma unroll
+    for (int f = 0; f < DM; ++f) {
+        xr0[f] = (v0 && f < d) ? a.x[(int64_t)j0 * d + f] : 0.0;
+        xr1[f] = (v1 && f < d) ? a.x[(int64_t)j1 * d + f] : 0.0;
     }
-    if (t == 0)
-        for (int f = 0; f < d; ++f) X[MR * d + f] = pv ? a.xs[p * d + f] : 0.0;
+    const double yn0 = v0 ? a.y[(int64_t)j0 * ny] : 0.0;
+    const double yn1 = v1 ? a.y[(int64_t)j1 * ny] : 0.0;
+#pragma unroll
+    for (int f = 0; f < DM; ++f) {
+        X[i0 * DM + f] = xr0[f];
+        X[i1 * DM + f] = xr1[f];
+    }
+    vbuf[i0] = yn0;
+    vbuf[i1] = yn1;
+    if (t == 0) {
+#pragma unroll
+        for (int f = 0; f < DM; ++f) X[MR * DM + f] = (pv && f < d) ? a.xs[p * d + f] : 0.0;
+        *yself = (pv && a.ys) ? a.ys[p * ny] : 0.0;
+    }
     const uint64_t b0 = __ballot(v0), b1 = __ballot(v1);
     const uint32_t vlo = (uint32_t)((b0 >> (seg * S)) & ((1ull << S) - 1ull));
     const uint32_t vhi = (uint32_t)((b1 >> (seg * S)) & ((1ull << S) - 1ull));
     __syncthreads();
 
-    // ---------------- fill: FP64 covariance of the lane's rows -------------
-    auto expo = [&](int i, int j) -> double {
-        const double* xi = X + i * d;
-        const double* xj = X + j * d;
+    // per-feature weights of the squared distance (ARD: h_f; isotropic: applied after)
+    const bool ard = kp.n_ls > 1;
+    double hw[DM];
+#pragma unroll
+    for (int f = 0; f < DM; ++f) hw[f] = (ard && f < d) ? kp.h[f] : 0.0;
+    // scaled squared distance between row coordinates xi and X[j]
+    auto sqdist = [&](const double (&xi)[DM], int j) -> double {
+        const double* xj = X + j * DM;
         double u = 0.0;
-        if (kp.n_ls == 1) {
-            for (int f = 0; f < d; ++f) {
+        if (!ard) {
+#pragma unroll
+            for (int f = 0; f < DM; ++f) {
                 const double tt = xi[f] - xj[f];
                 u = fma(tt, tt, u);
             }
             u *= kp.h[0];
         } else {
-            for (int f = 0; f < d; ++f) {
+#pragma unroll
+            for (int f = 0; f < DM; ++f) {
                 const double tt = xi[f] - xj[f];
-                u = fma(tt * tt, kp.h[f], u);
+                u = fma(tt * tt, hw[f], u);
             }
         }
-        return exp(-u);
+        return u;
     };
+
+    // ---------------- fill: FP64 covariance of the lane's rows -------------
     const double diag_add = kp.w + a.alpha;
     const double inv_s = kp.s != 0.0 ? 1.0 / kp.s : 0.0;
     // exponential of a valid off-diagonal pair from its covariance entry
     auto e_of = [&](double K) -> double { return kp.n_ls > 0 ? (K - kp.C) * inv_s : 0.0; };
-    // The lane's MR + 1 lower-triangle entries (i0, 0..i0), (i1, 0..i1), then its
-    // two rows against the point itself.
     double e0s = 0.0, e1s = 0.0;
 #pragma unroll 1
-    for (int e = 0; e < MR + 3; ++e) {
-        int i, j;
-        bool vi;
-        if (e <= i0) {
-            i = i0;
-            j = e;
-            vi = v0;
-        } else if (e <= MR) {
-            i = i1;
-            j = e - i0 - 1;
-            vi = v1;
-        } else {
-            i = e == MR + 1 ? i0 : i1;
-            j = MR;
-            vi = (e == MR + 1 ? v0 : v1) && pv;
-        }
-        const bool vj = j == MR ? true : row_valid<MR>(j, vlo, vhi);
-        const double Ev = (vi && vj && kp.n_ls > 0 && i != j) ? expo(i, j) : (i == j ? 1.0 : 0.0);
-        if (j < MR) {
-            double K = (vi && vj) ? fma(kp.s, Ev, kp.C) : 0.0;
-            if (i == j) K = vi ? K + diag_add : 1.0;
-            Kt[tri(i, j)] = K;
-        } else if (e == MR + 1) {
-            e0s = Ev;
-        } else {
-            e1s = Ev;
+    for (int e0 = 0; e0 < NE; e0 += U) {
+#pragma unroll
+        for (int uu = 0; uu < U; ++uu) {
+            const int e = e0 + uu;
+            if (e < NE) {
+                const SegEntry s = seg_entry<MR>(e, i0, i1);
+                const bool vi = s.first ? v0 : v1;
+                const bool vj = s.j == MR ? true : row_valid<MR>(s.j, vlo, vhi);
+                double xi[DM];
+#pragma unroll
+                for (int f = 0; f < DM; ++f) xi[f] = s.first ? xr0[f] : xr1[f];
+                const double u = sqdist(xi, s.j);
+                const double Ev = (vi && vj && kp.n_ls > 0 && s.i != s.j) ? exp(-u)
+                                                                         : (s.i == s.j ? 1.0 : 0.0);
+                if (e <= MR) {
+                    double K = (vi && vj) ? fma(kp.s, Ev, kp.C) : 0.0;
+                    if (s.i == s.j) K = vi ? K + diag_add : 1.0;
+                    Kt[tri(s.i, s.j)] = K;
+                } else if (e == MR + 1) {
+                    e0s = Ev;
+                } else {
+                    e1s = Ev;
+                }
+            }
         }
     }
     const double k0 = v0 ? fma(kp.s, e0s, kp.C) : 0.0;
@@ -321,8 +365,8 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
     // ---------------- outputs ----------------------------------------------
     double ll = 0.0, Gs = 0.0, at0 = 0.0, at1 = 0.0;
     for (int o = 0; o < ny; ++o) {
-        const double y0 = v0 ? a.y[(int64_t)j0 * ny + o] : 0.0;
-        const double y1 = v1 ? a.y[(int64_t)j1 * ny + o] : 0.0;
+        const double y0 = o == 0 ? vbuf[i0] : (v0 ? a.y[(int64_t)j0 * ny + o] : 0.0);
+        const double y1 = o == 0 ? vbuf[i1] : (v1 ? a.y[(int64_t)j1 * ny + o] : 0.0);
         double mu, av0 = 0.0, av1 = 0.0;
         if (full) {
             solve(y0, y1, av0, av1);
@@ -334,7 +378,7 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
         }
         if (live && a.mean && t == 0) a.mean[p * ny + o] = mu;
         if (a.ys) {
-            const double yo = pv ? a.ys[p * ny + o] : 0.0;
+            const double yo = o == 0 ? *yself : (pv ? a.ys[p * ny + o] : 0.0);
             const double res = yo - mu;
             ll += -0.5 * res * res / s2 - kHalfLog2PiS - log(sig);
             if (a.grad) {
@@ -347,91 +391,88 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
     if (live && a.pt && t == 0) a.pt[p * a.pt_stride] = ll;
 
     // ---------------- adjoint contraction ----------------------------------
+    // Over the lane's entries: pairs (i, j <= i) with weight W_ij (doubled off the
+    // diagonal) against dK_ij, and the self terms (i, point) against dk_i.
     if (a.grad && a.pt) {
         xv[i0] = bv0;
         xv[i1] = bv1;
         vbuf[i0] = at0;
         vbuf[i1] = at1;
         __syncthreads();
-        const bool ard = kp.n_ls > 1;
         double acc_w = 0.0, acc_C = 0.0, acc_s = 0.0, acc_l = 0.0;
-        double accf[kSegMaxDim];
+        double accf[DM];
 #pragma unroll
-        for (int f = 0; f < kSegMaxDim; ++f) accf[f] = 0.0;
-        const double* xs = X + MR * d;
-        // one row i of this lane: its self term and its pairs (i, j <= i)
-        auto row_terms = [&](int i, bool vi, double bi, double ati, double eis) {
-            if (!vi) return;
-            const double* xi = X + i * d;
-            const double v = ati - 2.0 * Gs * bi;
-            acc_C += v;
-            acc_s += v * eis;
-            if (kp.n_ls == 1) {
-                double u = 0.0;
-                for (int f = 0; f < d; ++f) {
-                    const double tt = xi[f] - xs[f];
-                    u = fma(tt, tt, u);
-                }
-                acc_l += v * eis * 2.0 * u * kp.h[0];
-            } else if (ard) {
-#pragma unroll
-                for (int f = 0; f < kSegMaxDim; ++f) {
-                    if (f < d) {
-                        const double tt = xi[f] - xs[f];
-                        accf[f] += v * eis * 2.0 * tt * tt * kp.h[f];
-                    }
-                }
-            }
+        for (int f = 0; f < DM; ++f) accf[f] = 0.0;
 #pragma unroll 1
-            for (int j = 0; j <= i; ++j) {
-                if (!row_valid<MR>(j, vlo, vhi)) continue;
-                const double bj = xv[j], aj = vbuf[j];
-                const double mult = j == i ? 1.0 : 2.0;
-                const double W = mult * (Gs * bi * bj - 0.5 * (bi * aj + ati * bj));
-                const double Ev = j == i ? 1.0 : e_of(Kt[tri(i, j)]);
-                if (j == i) acc_w += W;
-                acc_C += W;
-                acc_s += W * Ev;
-                const double* xj = X + j * d;
-                if (kp.n_ls == 1) {
-                    double u = 0.0;
-                    for (int f = 0; f < d; ++f) {
-                        const double tt = xi[f] - xj[f];
-                        u = fma(tt, tt, u);
-                    }
-                    acc_l += W * Ev * 2.0 * u * kp.h[0];
-                } else if (ard) {
+        for (int e0 = 0; e0 < NE; e0 += U) {
 #pragma unroll
-                    for (int f = 0; f < kSegMaxDim; ++f) {
-                        if (f < d) {
+            for (int uu = 0; uu < U; ++uu) {
+                const int e = e0 + uu;
+                if (e < NE) {
+                    const SegEntry s = seg_entry<MR>(e, i0, i1);
+                    const bool self = e > MR;  // (i, point)
+                    const int jr = self ? 0 : s.j;
+                    const bool vi = s.first ? v0 : v1;
+                    const bool valid = vi && (self || row_valid<MR>(s.j, vlo, vhi));
+                    const double bi = s.first ? bv0 : bv1;
+                    const double ati = s.first ? at0 : at1;
+                    const double bj = xv[jr], aj = vbuf[jr];
+                    const double mult = s.j == s.i ? 1.0 : 2.0;
+                    double W = self ? ati - 2.0 * Gs * bi
+                                    : mult * (Gs * bi * bj - 0.5 * (bi * aj + ati * bj));
+                    W = valid ? W : 0.0;
+                    const double Ev = self ? (s.first ? e0s : e1s)
+                                           : (s.j == s.i ? 1.0 : e_of(Kt[tri(s.i, jr)]));
+                    if (!self) acc_w += s.j == s.i ? W : 0.0;
+                    acc_C += W;
+                    const double WE = W * Ev;
+                    acc_s += WE;
+                    const double* xi = X + s.i * DM;  // (registers are saved for the factor)
+                    const double* xj = X + s.j * DM;
+                    if (!ard) {
+                        double u = 0.0;
+#pragma unroll
+                        for (int f = 0; f < DM; ++f) {
                             const double tt = xi[f] - xj[f];
-                            accf[f] += W * Ev * 2.0 * tt * tt * kp.h[f];
+                            u = fma(tt, tt, u);
+                        }
+                        acc_l = fma(WE * 2.0 * u, kp.h[0], acc_l);
+                    } else {
+#pragma unroll
+                        for (int f = 0; f < DM; ++f) {
+                            const double tt = xi[f] - xj[f];
+                            accf[f] = fma(WE * 2.0 * tt * tt, hw[f], accf[f]);
                         }
                     }
                 }
             }
-        };
-        row_terms(i0, v0, bv0, at0, e0s);
-        row_terms(i1, v1, bv1, at1, e1s);
+        }
         acc_w = seg_sum<S>(acc_w);
         acc_C = seg_sum<S>(acc_C);
         acc_s = seg_sum<S>(acc_s);
         acc_l = seg_sum<S>(acc_l);
-        double* gp = a.pt + p * a.pt_stride + 1;
-        if (live && t == 0) {
-            for (int q = 0; q < a.n_theta; ++q) gp[q] = 0.0;
-            if (kp.t_w >= 0) gp[kp.t_w] += kp.w * (acc_w + Gs);
-            if (kp.t_C >= 0) gp[kp.t_C] += kp.C * (acc_C + Gs);
-            if (kp.t_s >= 0) gp[kp.t_s] += kp.s * (acc_s + Gs);
-            if (kp.t_l >= 0 && kp.n_ls == 1) gp[kp.t_l] += kp.s * acc_l;
-        }
-        if (kp.t_l >= 0 && ard) {
+        if (ard) {
 #pragma unroll
-            for (int f = 0; f < kSegMaxDim; ++f) {
-                if (f < d) {
-                    const double af = seg_sum<S>(accf[f]);
-                    if (live && t == 0) gp[kp.t_l + f] += kp.s * af;
+            for (int f = 0; f < DM; ++f) accf[f] = seg_sum<S>(accf[f]);
+        }
+        // every gradient entry written once
+        if (live && t == 0) {
+            double* gp = a.pt + p * a.pt_stride + 1;
+            for (int q = 0; q < a.n_theta; ++q) {
+                double v = 0.0;
+                if (q == kp.t_w) v = kp.w * (acc_w + Gs);
+                else if (q == kp.t_C) v = kp.C * (acc_C + Gs);
+                else if (q == kp.t_s) v = kp.s * (acc_s + Gs);
+                else if (kp.t_l >= 0 && q >= kp.t_l && q < kp.t_l + (ard ? d : 1)) {
+                    if (!ard) {
+                        v = kp.s * acc_l;
+                    } else {
+#pragma unroll
+                        for (int f = 0; f < DM; ++f)
+                            if (q - kp.t_l == f) v = kp.s * accf[f];
+                    }
                 }
+                gp[q] = v;
             }
         }
     }
@@ -439,50 +480,57 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
 
 // One group of G points per block (no persistent loop: nothing loop-invariant is
 // kept live across groups).
-template <typename T, int MR>
-__global__ __launch_bounds__(64) void vecchia_seg_kernel(VecArgs a, RegExtra ex) {
+template <typename T, int MR, int DM>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+void vecchia_seg_kernel(VecArgs a, RegExtra ex) {
     constexpr int G = 64 / (MR / 2);
     const int64_t count = ex.pcount ? (int64_t)*ex.pcount : a.npts;
     const int64_t base = (int64_t)blockIdx.x * G;
-    if (base < count) seg_body<T, MR>(a, ex, base, count);
+    if (base < count) seg_body<T, MR, DM>(a, ex, base, count);
 }
 
 // Grid-stride form for point lists whose length is known on the device only.
-template <typename T, int MR>
+template <typename T, int MR, int DM>
 __global__ __launch_bounds__(64) void vecchia_seg_list_kernel(VecArgs a, RegExtra ex) {
     constexpr int G = 64 / (MR / 2);
     const int64_t count = (int64_t)*ex.pcount;
 #pragma unroll 1
     for (int64_t base = (int64_t)blockIdx.x * G; base < count; base += (int64_t)gridDim.x * G) {
-        seg_body<T, MR>(a, ex, base, count);
+        seg_body<T, MR, DM>(a, ex, base, count);
         __syncthreads();
     }
 }
 
-template <typename T, int MR>
-size_t seg_lds(int d) {
+template <typename T, int MR, int DM>
+size_t seg_lds() {
     constexpr int S = MR / 2;
     constexpr int G = 64 / S;
     constexpr int TRI = MR * (MR + 1) / 2;
-    const size_t doubles = (size_t)G * ((size_t)(MR + 1) * d + TRI + 2 * MR);
+    const size_t doubles = (size_t)G * ((size_t)(MR + 1) * DM + TRI + 2 * MR + 1);
     return doubles * sizeof(double) + (size_t)G * 2 * MR * sizeof(T);
 }
 
-template <typename T, int MR>
+template <typename T, int MR, int DM>
 void launch_seg_mr(const VecArgs& a, const RegExtra& ex, int64_t max_points, hipStream_t st) {
     constexpr int G = 64 / (MR / 2);
     if (ex.pcount) {
         // a device-side list: at most 1024 blocks striding over it
         int64_t blocks = (max_points + G - 1) / G;
         blocks = blocks < 1 ? 1 : (blocks > 1024 ? 1024 : blocks);
-        hipLaunchKernelGGL((vecchia_seg_list_kernel<T, MR>), dim3((unsigned)blocks), dim3(64),
-                           (seg_lds<T, MR>(a.d)), st, a, ex);
+        hipLaunchKernelGGL((vecchia_seg_list_kernel<T, MR, DM>), dim3((unsigned)blocks), dim3(64),
+                           (seg_lds<T, MR, DM>()), st, a, ex);
         return;
     }
     int64_t blocks = (max_points + G - 1) / G;
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL((vecchia_seg_kernel<T, MR>), dim3((unsigned)blocks), dim3(64),
-                       (seg_lds<T, MR>(a.d)), st, a, ex);
+    hipLaunchKernelGGL((vecchia_seg_kernel<T, MR, DM>), dim3((unsigned)blocks), dim3(64),
+                       (seg_lds<T, MR, DM>()), st, a, ex);
+}
+
+template <typename T, int MR>
+void launch_seg_d(const VecArgs& a, const RegExtra& ex, int64_t max_points, hipStream_t st) {
+    if (a.d <= 3) launch_seg_mr<T, MR, 3>(a, ex, max_points, st);
+    else launch_seg_mr<T, MR, kSegMaxDim>(a, ex, max_points, st);
 }
 
 }  // namespace
@@ -495,11 +543,11 @@ int launch_seg(const VecArgs& a, const RegExtra& ex, bool fp32, int64_t max_poin
                hipStream_t st) {
     if (!seg_supported(a.m, a.d)) return -1;
     if (fp32) {
-        if (a.m <= 16) launch_seg_mr<float, 16>(a, ex, max_points, st);
-        else launch_seg_mr<float, 32>(a, ex, max_points, st);
+        if (a.m <= 16) launch_seg_d<float, 16>(a, ex, max_points, st);
+        else launch_seg_d<float, 32>(a, ex, max_points, st);
     } else {
-        if (a.m <= 16) launch_seg_mr<double, 16>(a, ex, max_points, st);
-        else launch_seg_mr<double, 32>(a, ex, max_points, st);
+        if (a.m <= 16) launch_seg_d<double, 16>(a, ex, max_points, st);
+        else launch_seg_d<double, 32>(a, ex, max_points, st);
     }
     return 0;
 }
