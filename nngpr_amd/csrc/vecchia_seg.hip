@@ -37,11 +37,72 @@ constexpr int kSegMaxDim = 8;
 // above it are re-factored in FP64.
 constexpr double kSegPivotRatioMax = 1.0e3;
 
-template <int S>
-__device__ __forceinline__ double seg_sum(double v) {
-#pragma unroll
-    for (int off = S / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, S);
+// Lane exchanges inside a segment of S lanes (a 16-lane DPP row for S = 16, half
+// a row for S = 8) through DPP moves, 32 bits at a time.
+template <int CTRL>
+__device__ __forceinline__ float dpp_x(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_x(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+constexpr int kDppQuadSwap1 = 0xB1;     // quad_perm [1,0,3,2]
+constexpr int kDppQuadSwap2 = 0x4E;     // quad_perm [2,3,0,1]
+constexpr int kDppHalfMirror = 0x141;   // lane i <-> 7 - i within each 8 lanes
+constexpr int kDppMirror = 0x140;       // lane i <-> 15 - i within each 16 lanes
+constexpr int kDppRowBcast = 0x150;     // + n: lane n of each 16 lanes to all of them
+
+// Sum over the segment, the same value in every lane (each level adds two equal
+// partial sums in either order).
+template <int S, typename T>
+__device__ __forceinline__ T seg_sum(T v) {
+    static_assert(S == 8 || S == 16, "segments of 8 or 16 lanes");
+    v += dpp_x<kDppQuadSwap1>(v);
+    v += dpp_x<kDppQuadSwap2>(v);
+    v += dpp_x<kDppHalfMirror>(v);
+    if constexpr (S == 16) v += dpp_x<kDppMirror>(v);
     return v;
+}
+
+// Value of lane n (0 <= n < S, a constant once unrolled) of the segment.
+template <int S, typename T>
+__device__ __forceinline__ T seg_bcast(T v, int n, int lane) {
+    static_assert(S == 8 || S == 16, "segments of 8 or 16 lanes");
+    if constexpr (S == 16) {
+        switch (n) {
+            case 0: return dpp_x<kDppRowBcast + 0>(v);
+            case 1: return dpp_x<kDppRowBcast + 1>(v);
+            case 2: return dpp_x<kDppRowBcast + 2>(v);
+            case 3: return dpp_x<kDppRowBcast + 3>(v);
+            case 4: return dpp_x<kDppRowBcast + 4>(v);
+            case 5: return dpp_x<kDppRowBcast + 5>(v);
+            case 6: return dpp_x<kDppRowBcast + 6>(v);
+            case 7: return dpp_x<kDppRowBcast + 7>(v);
+            case 8: return dpp_x<kDppRowBcast + 8>(v);
+            case 9: return dpp_x<kDppRowBcast + 9>(v);
+            case 10: return dpp_x<kDppRowBcast + 10>(v);
+            case 11: return dpp_x<kDppRowBcast + 11>(v);
+            case 12: return dpp_x<kDppRowBcast + 12>(v);
+            case 13: return dpp_x<kDppRowBcast + 13>(v);
+            case 14: return dpp_x<kDppRowBcast + 14>(v);
+            default: return dpp_x<kDppRowBcast + 15>(v);
+        }
+    } else {
+        // lane n % 4 of every quad, then the other quad of the 8 if n / 4 differs
+        T q;
+        switch (n & 3) {
+            case 0: q = dpp_x<0x00>(v); break;   // quad_perm [0,0,0,0]
+            case 1: q = dpp_x<0x55>(v); break;   // quad_perm [1,1,1,1]
+            case 2: q = dpp_x<0xAA>(v); break;   // quad_perm [2,2,2,2]
+            default: q = dpp_x<0xFF>(v); break;  // quad_perm [3,3,3,3]
+        }
+        const T m = dpp_x<kDppHalfMirror>(q);
+        return ((lane >> 2) & 1) == (n >> 2) ? q : m;
+    }
 }
 
 // Validity of row j of this block: rows < S are bit j of vlo, rows >= S are bit
@@ -60,6 +121,70 @@ __device__ __forceinline__ int tri(int i, int j) { return i * (i + 1) / 2 + j; }
 __device__ __forceinline__ int opaque(int v) {
     asm volatile("" : "+v"(v));
     return v;
+}
+
+// L z = rhs for a lane's two rows, NR right-hand sides at once.  With the
+// strictly lower part of the rows in registers (L0: row t, L1: row MR - 1 - t) a
+// row's running value stops changing once its own step has passed, and z_i is
+// that value times 1 / L_ii.
+template <typename T, int MR, int NR>
+__device__ __forceinline__ void seg_fwd(const T (&L0)[MR / 2], const T (&L1)[MR], const T* dinv,
+                                        T dv0, T dv1, T (&r0)[NR], T (&r1)[NR], int lane) {
+    constexpr int S = MR / 2;
+#pragma unroll
+    for (int k = 0; k < MR; ++k) {
+        const int own = k < S ? k : MR - 1 - k;
+        const T dk = dinv[k];
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+            const T zk = seg_bcast<S>(k < S ? r0[q] : r1[q], own, lane) * dk;
+            if (k < S) r0[q] = fma(-L0[k < S ? k : 0], zk, r0[q]);
+            r1[q] = fma(-L1[k], zk, r1[q]);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+        r0[q] *= dv0;
+        r1[q] *= dv1;
+    }
+}
+
+// L^T x = z for a lane's two rows, NR right-hand sides at once (x starts at 0:
+// rows not yet solved add nothing through the cleared entries).  z in r0 / r1 is
+// replaced by x.
+template <typename T, int MR, int NR>
+__device__ __forceinline__ void seg_bwd(const T (&L0)[MR / 2], const T (&L1)[MR], const T* dinv,
+                                        T (&r0)[NR], T (&r1)[NR], int t, int lane) {
+    constexpr int S = MR / 2;
+    T x0[NR], x1[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+        x0[q] = (T)0;
+        x1[q] = (T)0;
+    }
+#pragma unroll
+    for (int k = MR - 1; k >= 0; --k) {
+        const int own = k < S ? k : MR - 1 - k;
+        const T dk = dinv[k];
+        T part[NR];
+#pragma unroll
+        for (int q = 0; q < NR; ++q)
+            part[q] = seg_sum<S>(fma(L1[k], x1[q], k < S ? L0[k < S ? k : 0] * x0[q] : (T)0));
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+            const T xk = (seg_bcast<S>(k < S ? r0[q] : r1[q], own, lane) - part[q]) * dk;
+            if (k < S) {
+                if (t == k) x0[q] = xk;
+            } else {
+                if (t == own) x1[q] = xk;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+        r0[q] = x0[q];
+        r1[q] = x1[q];
+    }
 }
 
 // Entry e of a lane (0 <= e < MR + 3): (i0, 0..i0), (i1, 0..i1), then (i0, MR) and
@@ -213,7 +338,7 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
     const double c = kp.s + kp.C + kp.w + (a.alpha_self ? a.alpha : 0.0);
     double dmax = fmax(v0 ? kp.s + kp.C + diag_add : 1.0, v1 ? kp.s + kp.C + diag_add : 1.0);
 #pragma unroll
-    for (int off = S / 2; off > 0; off >>= 1) dmax = fmax(dmax, __shfl_xor(dmax, off, S));
+    for (int off = S / 2; off > 0; off >>= 1) dmax = fmax(dmax, __shfl_xor(dmax, off, S));  // once per point
     __syncthreads();
 
     // ---------------- rows into registers (type T) --------------------------
@@ -283,47 +408,13 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
     }
     const bool live = pv && !bad && !ill;
 
-    // L z = rhs for the lane's rows: with the strictly lower part in registers a
-    // row's running value stops changing once its own step has passed, and z_i
-    // is that value times 1 / L_ii.
-    auto fwd = [&](T& r0, T& r1) {
-#pragma unroll
-        for (int k = 0; k < MR; ++k) {
-            const int own = k < S ? k : MR - 1 - k;
-            const T zk = __shfl(k < S ? r0 : r1, own, S) * dinv[k];
-            if (k < S) r0 = fma(-L0[k < S ? k : 0], zk, r0);
-            r1 = fma(-L1[k], zk, r1);
-        }
-        r0 *= dv0;
-        r1 *= dv1;
-    };
-    // L^T x = z for the lane's rows (x starts at 0: rows not yet solved add
-    // nothing through the cleared entries)
-    auto bwd = [&](T& r0, T& r1) {
-        T x0 = (T)0, x1 = (T)0;
-#pragma unroll
-        for (int k = MR - 1; k >= 0; --k) {
-            T part = fma(L1[k], x1, k < S ? L0[k < S ? k : 0] * x0 : (T)0);
-#pragma unroll
-            for (int off = S / 2; off > 0; off >>= 1) part += __shfl_xor(part, off, S);
-            const int own = k < S ? k : MR - 1 - k;
-            const T xk = (__shfl(k < S ? r0 : r1, own, S) - part) * dinv[k];
-            if (k < S) {
-                if (t == k) x0 = xk;
-            } else {
-                if (t == own) x1 = xk;
-            }
-        }
-        r0 = x0;
-        r1 = x1;
-    };
     // x = K^-1 rhs for the lane's rows; FP32 form: one FP64-residual refinement
     auto solve = [&](double rhs0, double rhs1, double& x0, double& x1) {
-        T u0 = (T)rhs0, u1 = (T)rhs1;
-        fwd(u0, u1);
-        bwd(u0, u1);
-        x0 = (double)u0;
-        x1 = (double)u1;
+        T u0[1] = {(T)rhs0}, u1[1] = {(T)rhs1};
+        seg_fwd<T, MR, 1>(L0, L1, dinv, dv0, dv1, u0, u1, lane);
+        seg_bwd<T, MR, 1>(L0, L1, dinv, u0, u1, t, lane);
+        x0 = (double)u0[0];
+        x1 = (double)u1[0];
         if constexpr (REFINE) {
             xv[i0] = x0;
             xv[i1] = x1;
@@ -338,44 +429,19 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
                 q1 = fma(-K1, xj, q1);
             }
             __syncthreads();
-            T w0 = (T)q0, w1 = (T)q1;
-            fwd(w0, w1);
-            bwd(w0, w1);
-            x0 += (double)w0;
-            x1 += (double)w1;
+            T w0[1] = {(T)q0}, w1[1] = {(T)q1};
+            seg_fwd<T, MR, 1>(L0, L1, dinv, dv0, dv1, w0, w1, lane);
+            seg_bwd<T, MR, 1>(L0, L1, dinv, w0, w1, t, lane);
+            x0 += (double)w0[0];
+            x1 += (double)w1[0];
         }
     };
 
-    // ---------------- sigma^2 ----------------------------------------------
-    const bool full = REFINE || (a.grad && a.ys);
-    double s2, bv0 = 0.0, bv1 = 0.0, zb0 = 0.0, zb1 = 0.0;
-    if (full) {
-        solve(k0, k1, bv0, bv1);
-        s2 = c - seg_sum<S>(k0 * bv0 + k1 * bv1);
-    } else {
-        T u0 = (T)k0, u1 = (T)k1;
-        fwd(u0, u1);
-        zb0 = (double)u0;
-        zb1 = (double)u1;
-        s2 = c - seg_sum<S>(zb0 * zb0 + zb1 * zb1);
-    }
-    const double sig = sqrt(s2);
-    if (live && a.stdv && t == 0) a.stdv[p] = sig;
-
-    // ---------------- outputs ----------------------------------------------
-    double ll = 0.0, Gs = 0.0, at0 = 0.0, at1 = 0.0;
-    for (int o = 0; o < ny; ++o) {
-        const double y0 = o == 0 ? vbuf[i0] : (v0 ? a.y[(int64_t)j0 * ny + o] : 0.0);
-        const double y1 = o == 0 ? vbuf[i1] : (v1 ? a.y[(int64_t)j1 * ny + o] : 0.0);
-        double mu, av0 = 0.0, av1 = 0.0;
-        if (full) {
-            solve(y0, y1, av0, av1);
-            mu = seg_sum<S>(k0 * av0 + k1 * av1);
-        } else {
-            T u0 = (T)y0, u1 = (T)y1;
-            fwd(u0, u1);
-            mu = seg_sum<S>(zb0 * (double)u0 + zb1 * (double)u1);
-        }
+    const bool with_grad = a.grad && a.ys;
+    double s2, sig, ll = 0.0, Gs = 0.0, at0 = 0.0, at1 = 0.0;
+    double bv0 = 0.0, bv1 = 0.0;
+    // one target column: mean, and the log-likelihood terms of the point
+    auto emit = [&](int o, double mu, double av0, double av1) {
         if (live && a.mean && t == 0) a.mean[p * ny + o] = mu;
         if (a.ys) {
             const double yo = o == 0 ? *yself : (pv ? a.ys[p * ny + o] : 0.0);
@@ -387,7 +453,52 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
                 at1 = fma(res / s2, av1, at1);
             }
         }
+    };
+    if constexpr (!REFINE) {
+        // z_k = L^-1 k and z_y = L^-1 y (first column) together: sigma^2 =
+        // c - z_k.z_k, mu = z_k.z_y; the gradient needs b = L^-T z_k, a = L^-T z_y
+        T r0[2] = {(T)k0, (T)vbuf[i0]}, r1[2] = {(T)k1, (T)vbuf[i1]};
+        seg_fwd<T, MR, 2>(L0, L1, dinv, dv0, dv1, r0, r1, lane);
+        const double zk0 = r0[0], zk1 = r1[0];
+        s2 = c - seg_sum<S>(zk0 * zk0 + zk1 * zk1);
+        sig = sqrt(s2);
+        const double mu0 = seg_sum<S>(zk0 * (double)r0[1] + zk1 * (double)r1[1]);
+        double av0 = 0.0, av1 = 0.0;
+        if (with_grad) {
+            seg_bwd<T, MR, 2>(L0, L1, dinv, r0, r1, t, lane);
+            bv0 = r0[0];
+            bv1 = r1[0];
+            av0 = r0[1];
+            av1 = r1[1];
+        }
+        emit(0, mu0, av0, av1);
+        for (int o = 1; o < ny; ++o) {
+            T u0[1] = {(T)(v0 ? a.y[(int64_t)j0 * ny + o] : 0.0)};
+            T u1[1] = {(T)(v1 ? a.y[(int64_t)j1 * ny + o] : 0.0)};
+            seg_fwd<T, MR, 1>(L0, L1, dinv, dv0, dv1, u0, u1, lane);
+            const double mu = seg_sum<S>(zk0 * (double)u0[0] + zk1 * (double)u1[0]);
+            double a0 = 0.0, a1 = 0.0;
+            if (with_grad) {
+                seg_bwd<T, MR, 1>(L0, L1, dinv, u0, u1, t, lane);
+                a0 = u0[0];
+                a1 = u1[0];
+            }
+            emit(o, mu, a0, a1);
+        }
+    } else {
+        // FP32 form: refined solves against the FP64 entries
+        solve(k0, k1, bv0, bv1);
+        s2 = c - seg_sum<S>(k0 * bv0 + k1 * bv1);
+        sig = sqrt(s2);
+        for (int o = 0; o < ny; ++o) {
+            const double y0 = o == 0 ? vbuf[i0] : (v0 ? a.y[(int64_t)j0 * ny + o] : 0.0);
+            const double y1 = o == 0 ? vbuf[i1] : (v1 ? a.y[(int64_t)j1 * ny + o] : 0.0);
+            double av0, av1;
+            solve(y0, y1, av0, av1);
+            emit(o, seg_sum<S>(k0 * av0 + k1 * av1), av0, av1);
+        }
     }
+    if (live && a.stdv && t == 0) a.stdv[p] = sig;
     if (live && a.pt && t == 0) a.pt[p * a.pt_stride] = ll;
 
     // ---------------- adjoint contraction ----------------------------------
