@@ -39,15 +39,17 @@ constexpr double kSegPivotRatioMax = 1.0e3;
 
 // Lane exchanges inside a segment of S lanes (a 16-lane DPP row for S = 16, half
 // a row for S = 8) through DPP moves, 32 bits at a time.
+// (Every pattern used here has a source lane for every lane, so bound_ctrl never
+// applies; it spares the moves that would initialise an `old` operand.)
 template <int CTRL>
 __device__ __forceinline__ float dpp_x(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xf, 0xf, true));
 }
 template <int CTRL>
 __device__ __forceinline__ double dpp_x(double v) {
     const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffLL), CTRL, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xf, 0xf, true);
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 constexpr int kDppQuadSwap1 = 0xB1;     // quad_perm [1,0,3,2]
@@ -103,6 +105,22 @@ __device__ __forceinline__ T seg_bcast(T v, int n, int lane) {
         const T m = dpp_x<kDppHalfMirror>(q);
         return ((lane >> 2) & 1) == (n >> 2) ? q : m;
     }
+}
+
+// 1 / sqrt(x) for x > 0: the hardware estimate and Newton steps
+// y <- y + y (1/2 - (x/2) y^2) (two in FP64, one in FP32).
+__device__ __forceinline__ double rsqrt_nr(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+    y = fma(y, fma(-h * y, y, 0.5), y);
+    y = fma(y, fma(-h * y, y, 0.5), y);
+    return y;
+}
+__device__ __forceinline__ float rsqrt_nr(float x) {
+    float y = __builtin_amdgcn_rsqf(x);
+    const float h = 0.5f * x;
+    y = fmaf(y, fmaf(-h * y, y, 0.5f), y);
+    return y;
 }
 
 // Validity of row j of this block: rows < S are bit j of vlo, rows >= S are bit
@@ -369,7 +387,7 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
         const T dkk = col[k];
         bad = bad || !(dkk > (T)0);
         pmin = fmin(pmin, (double)dkk);
-        const T inv = (T)1 / sqrt(dkk);
+        const T inv = rsqrt_nr(dkk);
         if (t == 0) dinv[k] = inv;
         if (k < S) {
             const T tr = L0[k] * inv * inv;
