@@ -42,6 +42,9 @@ struct GridMem {
     int32_t* sums;
     int32_t* cell_of;
     GridPt* pts;
+    // occupancy bits: row (c_1, c_2) holds ceil(g_0 / 64) words, bit x of word
+    // x / 64 set when cell x of the row has a ref
+    uint64_t* rowmask;
     int cap;
 };
 
@@ -72,6 +75,10 @@ struct GridSearchMem {
     GridQueryBins qb[kMaxGridBands + 1];
     int32_t* qcell_of;
     GridPt* qpts;
+    // search waves (64 queries each) in launch order, by estimated cost, and
+    // their costs
+    int32_t* wave_cost;
+    int32_t* wave_perm;
 };
 
 bool grid_supported(int d, int k);

@@ -45,11 +45,6 @@ constexpr int kScanBlock = 1024;   // items per block of the exclusive scan (256
 constexpr int kCoopWaves = 4;      // waves per search block
 constexpr int kCoopSlots = 16;     // LDS append slots per lane
 
-__global__ void grid_fill_i32_kernel(int32_t* p, int64_t n, int32_t v) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
 __device__ __forceinline__ bool point_finite(const double* u, int d) {
     bool fin = true;
     for (int f = 0; f < d; ++f) fin = fin && __builtin_isfinite(u[f]);
@@ -110,13 +105,11 @@ __global__ __launch_bounds__(256) void grid_bbox_kernel(const double* u64, int64
     }
 }
 
-// One thread: cell size and grid dimensions from the bounding box, for a grid
-// over n refs.  The cell size targets n / kPointsPerCell cells over the box; it
+// Cell size and grid dimensions from the bounding box, for a grid over n refs.  The cell size targets n / kPointsPerCell cells over the box; it
 // grows until the grid fits in `cap` cells.  Query tiles are 4 cells wide along
 // features with at least 8 cells; the tiled bucket count must fit in capq.
-__global__ void grid_setup_kernel(const long long* bb, int d, int64_t n, int cap, int capq,
-                                  GridParams* gp) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ void setup_grid(const long long* bb, int d, int64_t n, int cap, int capq,
+                           GridParams* gp) {
     double lo[3] = {0.0, 0.0, 0.0}, ext[3] = {0.0, 0.0, 0.0};
     bool any = true;
     for (int f = 0; f < d; ++f) {
@@ -190,22 +183,88 @@ __device__ __forceinline__ int cell_coord(double u, double lo, double inv_h, int
     return (int)c;
 }
 
-// Bucket of every point and the per-bucket counts.  Refs (tiled == 0): the cell in
-// row-major order.  Queries (tiled != 0): the cell in tile order.  A point with a
-// non-finite coordinate goes to bucket `nonfin`.
-__global__ void grid_count_kernel(const double* u64, int64_t n, int d, const GridParams* gp,
-                                  int tiled, int nonfin, int32_t* cnt, int32_t* bucket_of) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double* u = u64 + i * d;
-    int b = nonfin;
+// ---------------------------------------------------------------------------
+// Counting sorts of points into grid buckets, several grids per launch
+// ---------------------------------------------------------------------------
+// One counting sort: points = rows [row0, row0 + n) of u64, into nb buckets of
+// grid gp (the last bucket for points with a non-finite coordinate).  Refs
+// (tiled == 0): the bucket is the cell in row-major order, and the occupancy
+// words of the grid are filled.  Queries (tiled != 0): the cell in tile order.
+// t_off / f_off / s_off / w_off: first block of the job in the point, bucket,
+// scan and occupancy-word launches.
+struct SortJob {
+    GridParams* gp;
+    int32_t* cnt;
+    int32_t* start;
+    int32_t* fill;
+    int32_t* sums;
+    int32_t* bucket_of;
+    GridPt* pts;
+    uint64_t* rowmask;
+    int64_t n;
+    int64_t row0;
+    int32_t t_off, f_off, s_off, w_off;
+    int32_t nb;
+    int32_t tiled;
+    int32_t cap;
+    int32_t capq;
+};
+
+constexpr int kJobsPerLaunch = 15;  // keeps the argument block small
+
+struct SortArgs {
+    const double* u64;
+    const long long* bbox;
+    int d;
+    int n_jobs;
+    SortJob job[kJobsPerLaunch];
+};
+
+// Job of block b given each job's first block (field `off` of the jobs).
+template <int32_t SortJob::*OFF>
+__device__ __forceinline__ int job_of_block(const SortArgs& a, int64_t b) {
+    int j = 0;
+    for (int t = 1; t < a.n_jobs; ++t)
+        if ((int64_t)(a.job[t].*OFF) <= b) j = t;
+    return j;
+}
+
+__host__ __device__ inline int64_t scan_blocks(int64_t n) { return (n + kScanBlock - 1) / kScanBlock; }
+
+// Grid parameters of every ref job (one thread each).
+__global__ void sort_setup_kernel(SortArgs a) {
+    const int j = threadIdx.x;
+    if (j < a.n_jobs) {
+        const SortJob& J = a.job[j];
+        setup_grid(a.bbox, a.d, J.n, J.cap, J.capq, J.gp);
+    }
+}
+
+// Zeroes the counts and fill cursors (nb + 1 entries per job).
+__global__ __launch_bounds__(256) void sort_clear_kernel(SortArgs a) {
+    const SortJob& J = a.job[job_of_block<&SortJob::f_off>(a, blockIdx.x)];
+    const int64_t i = ((int64_t)blockIdx.x - J.f_off) * 256 + threadIdx.x;
+    if (i <= J.nb) {
+        J.cnt[i] = 0;
+        J.fill[i] = 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void sort_count_kernel(SortArgs a) {
+    const SortJob& J = a.job[job_of_block<&SortJob::t_off>(a, blockIdx.x)];
+    const int64_t i = ((int64_t)blockIdx.x - J.t_off) * 256 + threadIdx.x;
+    if (i >= J.n) return;
+    const int d = a.d;
+    const double* u = a.u64 + (J.row0 + i) * d;
+    const GridParams* gp = J.gp;
+    int b = J.nb - 1;
     if (point_finite(u, d)) {
         const double ih = gp->inv_h;
         int c[3] = {0, 0, 0};
 #pragma unroll
         for (int f = 0; f < 3; ++f)
             if (f < d) c[f] = cell_coord(u[f], gp->lo[f], ih, gp->g[f]);
-        if (tiled) {
+        if (J.tiled) {
             const int t0 = gp->t[0], t1 = gp->t[1], t2 = gp->t[2];
             const int tile = c[0] / t0 + gp->tg[0] * (c[1] / t1 + gp->tg[1] * (c[2] / t2));
             const int local = c[0] % t0 + t0 * (c[1] % t1 + t1 * (c[2] % t2));
@@ -214,26 +273,26 @@ __global__ void grid_count_kernel(const double* u64, int64_t n, int d, const Gri
             b = c[0] + gp->g[0] * (c[1] + gp->g[1] * c[2]);
         }
     }
-    bucket_of[i] = b;
-    atomicAdd(&cnt[b], 1);
+    J.bucket_of[i] = b;
+    atomicAdd(&J.cnt[b], 1);
 }
 
-// Point i (row idx_off + i) to its place in bucket order.
-__global__ void grid_scatter_kernel(const double* u64, int64_t n, int d, int64_t idx_off,
-                                    const int32_t* bucket_of, const int32_t* start, int32_t* fill,
-                                    GridPt* pts) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int c = bucket_of[i];
-    const int pos = start[c] + atomicAdd(&fill[c], 1);
+// Point i (row row0 + i) to its place in bucket order.
+__global__ __launch_bounds__(256) void sort_scatter_kernel(SortArgs a) {
+    const SortJob& J = a.job[job_of_block<&SortJob::t_off>(a, blockIdx.x)];
+    const int64_t i = ((int64_t)blockIdx.x - J.t_off) * 256 + threadIdx.x;
+    if (i >= J.n) return;
+    const int d = a.d;
+    const int c = J.bucket_of[i];
+    const int pos = J.start[c] + atomicAdd(&J.fill[c], 1);
     GridPt p;
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
         p.u[f] = 0.0;
-        if (f < d) p.u[f] = u64[i * d + f];
+        if (f < d) p.u[f] = a.u64[(J.row0 + i) * d + f];
     }
-    p.idx = idx_off + i;
-    pts[pos] = p;
+    p.idx = J.row0 + i;
+    J.pts[pos] = p;
 }
 
 // Exclusive prefix sum of the 256 threads' values; *total gets the block sum.
@@ -256,70 +315,110 @@ __device__ __forceinline__ int32_t block_excl_scan_256(int32_t v, int32_t* total
     return woff + x - v;
 }
 
-__global__ __launch_bounds__(256) void scan_local_kernel(const int32_t* in, int64_t n, int32_t* out,
-                                                         int32_t* sums) {
-    const int64_t base = (int64_t)blockIdx.x * kScanBlock + threadIdx.x * 4;
+// Exclusive scan of cnt into start (nb + 1 entries per job): per-block scans,
+// then each job's block sums (one block per job), then the block offsets.
+__global__ __launch_bounds__(256) void sort_scan_local_kernel(SortArgs a) {
+    const SortJob& J = a.job[job_of_block<&SortJob::s_off>(a, blockIdx.x)];
+    const int64_t blk = (int64_t)blockIdx.x - J.s_off;
+    const int64_t n = (int64_t)J.nb + 1;
+    const int64_t base = blk * kScanBlock + threadIdx.x * 4;
     int32_t v[4];
     int32_t s = 0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        v[i] = base + i < n ? in[base + i] : 0;
+        v[i] = base + i < n ? J.cnt[base + i] : 0;
         s += v[i];
     }
     int32_t tot;
     int32_t run = block_excl_scan_256(s, &tot);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        if (base + i < n) out[base + i] = run;
+        if (base + i < n) J.start[base + i] = run;
         run += v[i];
     }
-    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+    if (threadIdx.x == 0) J.sums[blk] = tot;
 }
 
-__global__ __launch_bounds__(256) void scan_sums_kernel(int32_t* sums, int nb) {
+__global__ __launch_bounds__(256) void sort_scan_sums_kernel(SortArgs a) {
+    const SortJob& J = a.job[blockIdx.x];
+    const int nb = (int)scan_blocks((int64_t)J.nb + 1);
     int32_t carry = 0;
     for (int c0 = 0; c0 < nb; c0 += 256) {
         const int i = c0 + threadIdx.x;
-        const int32_t v = i < nb ? sums[i] : 0;
+        const int32_t v = i < nb ? J.sums[i] : 0;
         int32_t tot;
         const int32_t ex = block_excl_scan_256(v, &tot);
-        if (i < nb) sums[i] = carry + ex;
+        if (i < nb) J.sums[i] = carry + ex;
         carry += tot;
     }
 }
 
-__global__ __launch_bounds__(256) void scan_add_kernel(int32_t* out, int64_t n, const int32_t* sums) {
-    const int64_t base = (int64_t)blockIdx.x * kScanBlock;
-    const int32_t off = sums[blockIdx.x];
+__global__ __launch_bounds__(256) void sort_scan_add_kernel(SortArgs a) {
+    const SortJob& J = a.job[job_of_block<&SortJob::s_off>(a, blockIdx.x)];
+    const int64_t blk = (int64_t)blockIdx.x - J.s_off;
+    const int64_t n = (int64_t)J.nb + 1;
+    const int64_t base = blk * kScanBlock;
+    const int32_t off = J.sums[blk];
     for (int i = threadIdx.x; i < kScanBlock; i += 256)
-        if (base + i < n) out[base + i] += off;
+        if (base + i < n) J.start[base + i] += off;
 }
 
-int64_t scan_blocks(int64_t n) { return (n + kScanBlock - 1) / kScanBlock; }
+// Occupancy words of the ref grids: one thread per word (grid dimensions read
+// on the device; threads past the grid's last word return).
+__global__ __launch_bounds__(256) void sort_rowmask_kernel(SortArgs a) {
+    const SortJob& J = a.job[job_of_block<&SortJob::w_off>(a, blockIdx.x)];
+    const int64_t w = ((int64_t)blockIdx.x - J.w_off) * 256 + threadIdx.x;
+    const GridParams* gp = J.gp;
+    const int g0 = gp->g[0];
+    const int wpr = (g0 + 63) >> 6;
+    const int64_t rows = (int64_t)gp->g[1] * gp->g[2];
+    if (w >= rows * wpr) return;
+    const int64_t row = w / wpr;
+    const int x0 = (int)(w % wpr) * 64;
+    const int xn = min(64, g0 - x0);
+    const int32_t* st = J.start + row * g0 + x0;
+    uint64_t m = 0;
+    for (int b = 0; b < xn; ++b)
+        if (st[b + 1] > st[b]) m |= 1ull << b;
+    J.rowmask[w] = m;
+}
 
-// Counting sort of n points (rows idx_off ...) into nb buckets (the last, nb - 1,
-// for non-finite points); start gets nb + 1 entries.
-void bin_points(const double* u64, int64_t n, int d, int64_t idx_off, const GridParams* gp,
-                int tiled, int nb, int32_t* cnt, int32_t* start, int32_t* fill, int32_t* sums,
-                int32_t* bucket_of, GridPt* pts, hipStream_t st) {
-    const int64_t ns = (int64_t)nb + 1;
-    const unsigned gz = (unsigned)((ns + 255) / 256);
-    hipLaunchKernelGGL(grid_fill_i32_kernel, dim3(gz), dim3(256), 0, st, cnt, ns, 0);
-    hipLaunchKernelGGL(grid_fill_i32_kernel, dim3(gz), dim3(256), 0, st, fill, ns, 0);
-    if (n > 0) {
-        const unsigned gn = (unsigned)((n + 255) / 256);
-        hipLaunchKernelGGL(grid_count_kernel, dim3(gn), dim3(256), 0, st, u64, n, d, gp, tiled,
-                           nb - 1, cnt, bucket_of);
-    }
-    const int64_t nsb = scan_blocks(ns);
-    hipLaunchKernelGGL(scan_local_kernel, dim3((unsigned)nsb), dim3(256), 0, st, cnt, ns, start,
-                       sums);
-    hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(256), 0, st, sums, (int)nsb);
-    hipLaunchKernelGGL(scan_add_kernel, dim3((unsigned)nsb), dim3(256), 0, st, start, ns, sums);
-    if (n > 0) {
-        const unsigned gn = (unsigned)((n + 255) / 256);
-        hipLaunchKernelGGL(grid_scatter_kernel, dim3(gn), dim3(256), 0, st, u64, n, d, idx_off,
-                           bucket_of, start, fill, pts);
+// Occupancy words of a grid of at most `cap` cells: every row holds
+// ceil(g_0 / 64) <= g_0 / 64 + 1 words, over at most `cap` rows.
+size_t rowmask_words_max(int cap) { return (size_t)cap + (size_t)cap / 64 + 2; }
+
+// Runs the counting sorts of `jobs` (refs: grid setup first and occupancy words
+// after), kJobsPerLaunch jobs per launch.
+void run_sorts(const double* u64, const long long* bbox, int d, const SortJob* jobs, int n_jobs,
+               bool refs, hipStream_t st) {
+    for (int j0 = 0; j0 < n_jobs; j0 += kJobsPerLaunch) {
+        SortArgs a{};
+        a.u64 = u64;
+        a.bbox = bbox;
+        a.d = d;
+        a.n_jobs = std::min(kJobsPerLaunch, n_jobs - j0);
+        int64_t tb = 0, fb = 0, sb = 0, wb = 0;
+        for (int t = 0; t < a.n_jobs; ++t) {
+            SortJob J = jobs[j0 + t];
+            J.t_off = (int32_t)tb;
+            J.f_off = (int32_t)fb;
+            J.s_off = (int32_t)sb;
+            J.w_off = (int32_t)wb;
+            tb += (J.n + 255) / 256;
+            fb += ((int64_t)J.nb + 1 + 255) / 256;
+            sb += scan_blocks((int64_t)J.nb + 1);
+            if (refs) wb += (int64_t)(rowmask_words_max(J.cap) + 255) / 256;
+            a.job[t] = J;
+        }
+        if (refs) hipLaunchKernelGGL(sort_setup_kernel, dim3(1), dim3(64), 0, st, a);
+        hipLaunchKernelGGL(sort_clear_kernel, dim3((unsigned)fb), dim3(256), 0, st, a);
+        if (tb > 0) hipLaunchKernelGGL(sort_count_kernel, dim3((unsigned)tb), dim3(256), 0, st, a);
+        hipLaunchKernelGGL(sort_scan_local_kernel, dim3((unsigned)sb), dim3(256), 0, st, a);
+        hipLaunchKernelGGL(sort_scan_sums_kernel, dim3((unsigned)a.n_jobs), dim3(256), 0, st, a);
+        hipLaunchKernelGGL(sort_scan_add_kernel, dim3((unsigned)sb), dim3(256), 0, st, a);
+        if (tb > 0)
+            hipLaunchKernelGGL(sort_scatter_kernel, dim3((unsigned)tb), dim3(256), 0, st, a);
+        if (refs) hipLaunchKernelGGL(sort_rowmask_kernel, dim3((unsigned)wb), dim3(256), 0, st, a);
     }
 }
 
@@ -331,6 +430,7 @@ void bin_points(const double* u64, int64_t n, int d, int64_t idx_off, const Grid
 struct BandDesc {
     const GridParams* gp;
     const int32_t* cell_start;  // ref cells, row-major
+    const uint64_t* rowmask;    // occupancy bits of the ref cells
     const GridPt* rpts;         // refs in cell order
     const GridPt* qpts;         // the band's queries in tile order
     int64_t n_q;
@@ -340,6 +440,8 @@ struct BandDesc {
 struct SearchArgs {
     int n_bands;
     BandDesc band[kMaxGridBands + 1];
+    int64_t n_waves;          // waves over all bands
+    const int32_t* perm;      // launch slot -> wave (most expensive first)
     int64_t n_ref;
     int has_cut;
     int64_t cut;
@@ -398,11 +500,15 @@ __attribute__((amdgpu_waves_per_eu(K <= 16 ? 3 : 2)))
 void lane_knn_kernel(SearchArgs a) {
     constexpr int B = kCoopSlots;
     constexpr int CH = 4;  // refs per step of a lane
+    constexpr int NS = 4;  // segments examined per segment step of a lane
     __shared__ double sk[kCoopWaves][B][64];
     __shared__ int si[kCoopWaves][B][64];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
     const int lane = threadIdx.x & 63;
-    const int64_t wg = (int64_t)blockIdx.x * kCoopWaves + w;
+    const int64_t slot = (int64_t)blockIdx.x * kCoopWaves + w;
+    // slots past the last wave map past every band (no query)
+    const int64_t wg = slot < a.n_waves ? (int64_t)__builtin_amdgcn_readfirstlane(a.perm[slot])
+                                        : a.n_waves;
     int bj = 0;
     for (int t = 1; t < a.n_bands; ++t)
         if (a.band[t].wave_off <= wg) bj = t;
@@ -474,6 +580,58 @@ void lane_knn_kernel(SearchArgs a) {
             nseg = 2 * ny * (z1 - z0 + 1);
         };
         ring_bounds();
+        const uint64_t* rmask = bd0.rowmask;
+        const int wpr = (g[0] + 63) >> 6;
+        // Cell run of segment s (of row (y, z)) of the current ring (empty:
+        // ca > cb): row c_1 + g_1 c_2, first and last cell, clipped to the ball of squared radius rr
+        // around the query.  A ref outside that ball has key > tk (rr includes
+        // the slack tol), so it would not be admitted; cells [x_lo, x_hi] hold
+        // every ref whose x lies in [q_x - rx, q_x + rx] (same floor map as the
+        // binning).
+        auto seg_run = [&](int s, int y, int z, int& row, int& ca, int& cb) {
+            ca = 1;
+            cb = 0;
+            row = 0;
+            if (s >= nseg) return;
+            const int part = s & 1;
+            row = y + g[1] * z;
+            const bool full = R == 0 || z == c[2] - R || z == c[2] + R || y == c[1] - R ||
+                              y == c[1] + R;
+            if (full) {
+                if (part == 0) {
+                    ca = x0;
+                    cb = x1;
+                }
+            } else {
+                const int xc = part == 0 ? c[0] - R : c[0] + R;
+                if (xc >= 0 && xc < g[0]) ca = cb = xc;
+            }
+            if ((ca <= cb) & (rr < __builtin_inf())) {
+                double s2 = 0.0;
+                if (D >= 2) {
+                    const double ylo = lo[1] + (double)y * h;
+                    const double dy = fmax(fmax(ylo - qu[1], qu[1] - (ylo + h)), 0.0);
+                    s2 = dy * dy;
+                }
+                if (D >= 3) {
+                    const double zlo = lo[2] + (double)z * h;
+                    const double dz = fmax(fmax(zlo - qu[2], qu[2] - (zlo + h)), 0.0);
+                    s2 = fma(dz, dz, s2);
+                }
+                const double rem = rr - s2;
+                if (rem < 0.0) {
+                    ca = 1;
+                    cb = 0;
+                } else {
+                    const double rx = sqrt(rem);
+                    const double gx = (double)g[0];
+                    const double fl = fmin(fmax(floor((qu[0] - rx - lo[0]) * inv_h), -1.0), gx);
+                    const double fh = fmin(fmax(floor((qu[0] + rx - lo[0]) * inv_h), -1.0), gx);
+                    ca = max(ca, (int)fl);
+                    cb = min(cb, (int)fh);
+                }
+            }
+        };
         int32_t p = 0, e = 0;
         bool ring_end = false;   // ring walked: wait for the merge, then the stop test
         bool done = !active;
@@ -559,56 +717,57 @@ void lane_knn_kernel(SearchArgs a) {
                 }
                 p += CH;
             } else if (sg < nseg) {
-                // next segment of the ring
-                const int r = sg >> 1, part = sg & 1;
-                ++sg;
-                const int y = y0 + r % ny, z = z0 + r / ny;
-                const int base = g[0] * (y + g[1] * z);
-                const bool full = R == 0 || z == c[2] - R || z == c[2] + R || y == c[1] - R ||
-                                  y == c[1] + R;
-                int cx0 = 1, cx1 = 0;  // empty
-                if (full) {
-                    if (part == 0) {
-                        cx0 = x0;
-                        cx1 = x1;
-                    }
-                } else {
-                    const int xc = part == 0 ? c[0] - R : c[0] + R;
-                    if (xc >= 0 && xc < g[0]) cx0 = cx1 = xc;
+                // The next NS segments of the ring at once: their cell runs (clipped
+                // to the admission ball) and the occupancy word of each run; the
+                // first run with a ref is loaded, the empty ones before it skipped.
+                int ca[NS], cb[NS], crow[NS];
+                uint64_t mk[NS];
+                // rows of the ring run y fastest: row r is (y0 + r % ny, z0 + r / ny)
+                const int r0 = sg >> 1;
+                const int rz = r0 / ny;
+                const int ry = r0 - rz * ny;
+#pragma unroll
+                for (int u = 0; u < NS; ++u) {
+                    const int dr = ((sg & 1) + u) >> 1;
+                    int yy = ry + dr, zz = rz;
+#pragma unroll
+                    for (int wrap = 0; wrap < (NS + 1) / 2; ++wrap)
+                        if (yy >= ny) {
+                            yy -= ny;
+                            ++zz;
+                        }
+                    seg_run(sg + u, y0 + yy, z0 + zz, crow[u], ca[u], cb[u]);
+                    mk[u] = ca[u] <= cb[u] ? rmask[(int64_t)crow[u] * wpr + (ca[u] >> 6)] : 0ull;
                 }
-                // Clip to the ball of squared radius rr around the query: a ref
-                // outside it has key > tk (rr includes the slack tol), so it would
-                // not be admitted.  Cells [x_lo, x_hi] hold every ref whose x lies
-                // in [q_x - rx, q_x + rx] (same floor map as the binning).
-                if ((cx0 <= cx1) & (rr < __builtin_inf())) {
-                    double s2 = 0.0;
-                    if (D >= 2) {
-                        const double ylo = lo[1] + (double)y * h;
-                        const double dy = fmax(fmax(ylo - qu[1], qu[1] - (ylo + h)), 0.0);
-                        s2 = dy * dy;
+                int pick = NS;
+#pragma unroll
+                for (int u = NS - 1; u >= 0; --u) {
+                    bool any = false;
+                    if (ca[u] <= cb[u]) {
+                        if ((ca[u] >> 6) != (cb[u] >> 6)) {
+                            any = true;  // run over several words: not checked
+                        } else {
+                            const int wdt = cb[u] - ca[u] + 1;
+                            const uint64_t sel = wdt >= 64 ? ~0ull : ((1ull << wdt) - 1ull);
+                            any = ((mk[u] >> (ca[u] & 63)) & sel) != 0ull;
+                        }
                     }
-                    if (D >= 3) {
-                        const double zlo = lo[2] + (double)z * h;
-                        const double dz = fmax(fmax(zlo - qu[2], qu[2] - (zlo + h)), 0.0);
-                        s2 = fma(dz, dz, s2);
-                    }
-                    const double rem = rr - s2;
-                    if (rem < 0.0) {
-                        cx0 = 1;
-                        cx1 = 0;
-                    } else {
-                        const double rx = sqrt(rem);
-                        const double gx = (double)g[0];
-                        const double fl = fmin(fmax(floor((qu[0] - rx - lo[0]) * inv_h), -1.0), gx);
-                        const double fh = fmin(fmax(floor((qu[0] + rx - lo[0]) * inv_h), -1.0), gx);
-                        cx0 = max(cx0, (int)fl);
-                        cx1 = min(cx1, (int)fh);
-                    }
+                    if (any) pick = u;
                 }
-                if (cx0 <= cx1) {
-                    p = cs[base + cx0];
-                    e = cs[base + cx1 + 1];
+                int xa = 1, xb = 0, xrow = 0;
+#pragma unroll
+                for (int u = 0; u < NS; ++u)
+                    if (pick == u) {
+                        xa = ca[u];
+                        xb = cb[u];
+                        xrow = crow[u];
+                    }
+                if (pick < NS) {
+                    const int xbase = g[0] * xrow;
+                    p = cs[xbase + xa];
+                    e = cs[xbase + xb + 1];
                 }
+                sg = min(sg + (pick < NS ? pick + 1 : NS), nseg);
             } else {
                 ring_end = true;
             }
@@ -619,20 +778,91 @@ void lane_knn_kernel(SearchArgs a) {
     if (jv) store_list<K>(ids, pad_out, a.out_idx + q * a.k);
 }
 
+// Estimated cost of each search wave: the largest number of refs in the 3 x 3 x 3
+// cells around one of its queries.
+__global__ __launch_bounds__(64 * kCoopWaves) void wave_cost_kernel(SearchArgs a,
+                                                                    int32_t* cost) {
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int64_t wg = (int64_t)blockIdx.x * kCoopWaves + w;
+    if (wg >= a.n_waves) return;
+    int bj = 0;
+    for (int t = 1; t < a.n_bands; ++t)
+        if (a.band[t].wave_off <= wg) bj = t;
+    const BandDesc& bd = a.band[bj];
+    const int64_t j = (wg - bd.wave_off) * 64 + lane;
+    int c = 0;
+    if (j < bd.n_q) {
+        const GridPt qp = bd.qpts[j];
+        const GridParams& P = *bd.gp;
+        bool fin = true;
+        int cc[3];
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            fin = fin && __builtin_isfinite(qp.u[f]);
+            cc[f] = fin ? cell_coord(qp.u[f], P.lo[f], P.inv_h, P.g[f]) : 0;
+        }
+        if (fin) {
+            const int x0 = max(cc[0] - 1, 0), x1 = min(cc[0] + 1, P.g[0] - 1);
+            for (int z = max(cc[2] - 1, 0); z <= min(cc[2] + 1, P.g[2] - 1); ++z)
+                for (int y = max(cc[1] - 1, 0); y <= min(cc[1] + 1, P.g[1] - 1); ++y) {
+                    const int base = P.g[0] * (y + P.g[1] * z);
+                    c += bd.cell_start[base + x1 + 1] - bd.cell_start[base + x0];
+                }
+        }
+    }
+    c = wave_max32(c);
+    if (lane == 0) cost[wg] = c;
+}
+
+// One block: counting sort of the waves by cost bucket (4 per doubling),
+// most expensive first (order inside a bucket is arbitrary).
+constexpr int kCostBuckets = 64;
+__global__ __launch_bounds__(1024) void wave_sort_kernel(const int32_t* cost, int64_t n,
+                                                        int32_t* perm) {
+    __shared__ int hist[kCostBuckets];
+    if (threadIdx.x < kCostBuckets) hist[threadIdx.x] = 0;
+    __syncthreads();
+    auto bucket = [](int c) {
+        const int b = (int)(4.0f * __log2f(1.0f + (float)c));
+        return kCostBuckets - 1 - min(max(b, 0), kCostBuckets - 1);
+    };
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&hist[bucket(cost[i])], 1);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int b = 0; b < kCostBuckets; ++b) {
+            const int v = hist[b];
+            hist[b] = run;
+            run += v;
+        }
+    }
+    __syncthreads();
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
+        perm[atomicAdd(&hist[bucket(cost[i])], 1)] = (int32_t)i;
+}
+
 template <int D, int K>
 void launch_search_dk(const SearchArgs& c, unsigned blocks, hipStream_t st) {
     hipLaunchKernelGGL((lane_knn_kernel<D, K>), dim3(blocks), dim3(64 * kCoopWaves), 0, st, c);
 }
 
 // One launch over every band's waves.
-void launch_search(SearchArgs& c, int d, hipStream_t st) {
+// Waves are launched most expensive first, so that long waves do not start last.
+void launch_search(SearchArgs& c, int d, const GridSearchMem& M, hipStream_t st) {
     int64_t waves = 0;
     for (int t = 0; t < c.n_bands; ++t) {
         c.band[t].wave_off = waves;
         waves += (c.band[t].n_q + 63) / 64;
     }
     if (waves == 0) return;
+    c.n_waves = waves;
+    c.perm = M.wave_perm;
     const unsigned blocks = (unsigned)((waves + kCoopWaves - 1) / kCoopWaves);
+    hipLaunchKernelGGL(wave_cost_kernel, dim3(blocks), dim3(64 * kCoopWaves), 0, st, c,
+                       M.wave_cost);
+    hipLaunchKernelGGL(wave_sort_kernel, dim3(1), dim3(1024), 0, st, M.wave_cost, waves,
+                       M.wave_perm);
     const int K = c.k <= 16 ? 16 : (c.k <= 32 ? 32 : 64);
     switch (d * 100 + K) {
         case 116: launch_search_dk<1, 16>(c, blocks, st); break;
@@ -666,30 +896,56 @@ void carve_grid(Carver& c, int64_t n_pts, GridMem* M) {
     M->sums = c.take<int32_t>((size_t)scan_blocks(ns) + 1);
     M->cell_of = c.take<int32_t>((size_t)n_pts + 1);
     M->pts = c.take<GridPt>((size_t)n_pts + 1);
+    M->rowmask = c.take<uint64_t>(rowmask_words_max(cap));
 }
 
-// Grid over the first n_pts scaled refs of u64, with the bounding box of all refs.
-void build_grid(const double* u64, int64_t n_pts, int d, const long long* bbox, const GridMem& M,
-                hipStream_t st) {
-    hipLaunchKernelGGL(grid_setup_kernel, dim3(1), dim3(64), 0, st, bbox, d, n_pts, M.cap,
-                       grid_capq(M.cap), M.params);
-    bin_points(u64, n_pts, d, 0, M.params, 0, M.cap + 1, M.cnt, M.start, M.fill, M.sums,
-               M.cell_of, M.pts, st);
+// Counting-sort job of a grid over the first n_pts refs.
+SortJob ref_job(const GridMem& M, int64_t n_pts) {
+    SortJob J{};
+    J.gp = M.params;
+    J.cnt = M.cnt;
+    J.start = M.start;
+    J.fill = M.fill;
+    J.sums = M.sums;
+    J.bucket_of = M.cell_of;
+    J.pts = M.pts;
+    J.rowmask = M.rowmask;
+    J.n = n_pts;
+    J.row0 = 0;
+    J.nb = M.cap + 1;
+    J.tiled = 0;
+    J.cap = M.cap;
+    J.capq = grid_capq(M.cap);
+    return J;
 }
 
-// Sorts query rows [qa, qb) by tile of grid M into `bins` and qpts[pos ...] and
-// describes the band.
-BandDesc bin_band(const GridMem& M, const double* qu64, int64_t qa, int64_t qb, int64_t pos, int d,
-                  const GridQueryBins& bins, const GridSearchMem& S, hipStream_t st) {
-    BandDesc b{};
-    b.gp = M.params;
-    b.cell_start = M.start;
-    b.rpts = M.pts;
-    b.qpts = S.qpts + pos;
-    b.n_q = qb - qa;
-    bin_points(qu64 + qa * d, qb - qa, d, qa, M.params, 1, bins.capq + 1, bins.cnt, bins.start,
-               bins.fill, bins.sums, S.qcell_of + pos, S.qpts + pos, st);
-    return b;
+// Counting-sort job of query rows [qa, qb) into the tiles of grid M, written to
+// qpts[pos ...], and the band it describes.
+SortJob query_job(const GridMem& M, int64_t qa, int64_t qb, int64_t pos,
+                  const GridQueryBins& bins, const GridSearchMem& S, BandDesc* band) {
+    SortJob J{};
+    J.gp = M.params;
+    J.cnt = bins.cnt;
+    J.start = bins.start;
+    J.fill = bins.fill;
+    J.sums = bins.sums;
+    J.bucket_of = S.qcell_of + pos;
+    J.pts = S.qpts + pos;
+    J.rowmask = nullptr;
+    J.n = qb - qa;
+    J.row0 = qa;
+    J.nb = bins.capq + 1;
+    J.tiled = 1;
+    J.cap = M.cap;
+    J.capq = bins.capq;
+    *band = BandDesc{};
+    band->gp = M.params;
+    band->cell_start = M.start;
+    band->rowmask = M.rowmask;
+    band->rpts = M.pts;
+    band->qpts = S.qpts + pos;
+    band->n_q = qb - qa;
+    return J;
 }
 
 void carve_bins(Carver& c, int capq, GridQueryBins* Q) {
@@ -706,8 +962,8 @@ void carve_bins(Carver& c, int capq, GridQueryBins* Q) {
 bool grid_supported(int d, int k) { return d >= 1 && d <= 3 && k >= 1 && k <= 64; }
 
 int64_t grid_prefix_limit(int64_t n_ref, int k) {
-    return std::max<int64_t>(std::max<int64_t>(64, 2 * (int64_t)k),
-                             std::min<int64_t>(2048, n_ref / 16));
+    (void)n_ref;
+    return std::max<int64_t>(64, 2 * (int64_t)k);
 }
 
 void carve_grid_refs(Carver& c, int64_t n_ref, GridRefMem* M) {
@@ -731,6 +987,10 @@ void carve_grid_search(Carver& c, int64_t n_ref, int64_t n_query, int k, bool wi
     carve_bins(c, grid_capq(grid_cap(n_ref)), &M->qb[nl]);
     M->qcell_of = c.take<int32_t>((size_t)n_query + 1);
     M->qpts = c.take<GridPt>((size_t)n_query + 1);
+    // every band adds at most one partial wave
+    const size_t max_waves = (size_t)(n_query / 64) + (size_t)nl + 2;
+    M->wave_cost = c.take<int32_t>(max_waves);
+    M->wave_perm = c.take<int32_t>(max_waves);
 }
 
 int grid_build(const double* u64, int64_t n, int d, const GridRefMem& M, hipStream_t st) {
@@ -741,7 +1001,8 @@ int grid_build(const double* u64, int64_t n, int d, const GridRefMem& M, hipStre
         const unsigned gb = (unsigned)std::min<int64_t>(512, (n + 255) / 256);
         hipLaunchKernelGGL(grid_bbox_kernel, dim3(gb), dim3(256), 0, st, u64, n, d, M.bbox);
     }
-    build_grid(u64, n, d, M.bbox, M.full, st);
+    const SortJob J = ref_job(M.full, n);
+    run_sorts(u64, M.bbox, d, &J, 1, true, st);
     timing_end(TK_KNN_GRID_BUILD, st);
     NNGP_HIP_CHECK(hipGetLastError());
     return NNGP_OK;
@@ -758,8 +1019,10 @@ int grid_search(const GridRefMem& G, const double* ru64, int64_t n_ref, int d,
     sa.k = k;
     sa.out_idx = out_idx;
     timing_begin(TK_KNN_GRID_BUILD, st);
+    SortJob rjobs[kMaxGridBands], qjobs[kMaxGridBands + 1];
+    int n_r = 0, n_q = 0;
     if (!has_cut) {
-        sa.band[0] = bin_band(G.full, qu64, 0, n_query, 0, d, M.qb[M.n_lower], M, st);
+        qjobs[n_q++] = query_job(G.full, 0, n_query, 0, M.qb[M.n_lower], M, &sa.band[0]);
         sa.n_bands = 1;
     } else {
         // Band j: rows whose limit q + cut lies in [L, 2L), L = L0 * 2^j; the top
@@ -775,18 +1038,23 @@ int grid_search(const GridRefMem& G, const double* ru64, int64_t n_ref, int d,
             if (qb > qa) {
                 const GridMem* gm = &G.full;
                 if (!top) {
-                    build_grid(ru64, 2 * L, d, G.bbox, M.band[j], st);
+                    rjobs[n_r++] = ref_job(M.band[j], 2 * L);
                     gm = &M.band[j];
                 }
-                sa.band[sa.n_bands++] =
-                    bin_band(*gm, qu64, qa, qb, qa - row0, d, M.qb[top ? M.n_lower : j], M, st);
+                qjobs[n_q++] = query_job(*gm, qa, qb, qa - row0, M.qb[top ? M.n_lower : j], M,
+                                         &sa.band[sa.n_bands]);
+                ++sa.n_bands;
             }
             if (top) break;
         }
     }
+    // grids over the lower bands' ref prefixes (bounding box of all refs), then
+    // every band's queries into tile order
+    if (n_r > 0) run_sorts(ru64, G.bbox, d, rjobs, n_r, true, st);
+    run_sorts(qu64, G.bbox, d, qjobs, n_q, false, st);
     timing_end(TK_KNN_GRID_BUILD, st);
     timing_begin(TK_KNN_GRID, st);
-    launch_search(sa, d, st);
+    launch_search(sa, d, M, st);
     timing_end(TK_KNN_GRID, st);
     NNGP_HIP_CHECK(hipGetLastError());
     return NNGP_OK;
