@@ -500,7 +500,6 @@ __attribute__((amdgpu_waves_per_eu(K <= 16 ? 3 : 2)))
 void lane_knn_kernel(SearchArgs a) {
     constexpr int B = kCoopSlots;
     constexpr int CH = 4;  // refs per step of a lane
-    constexpr int NS = 4;  // segments examined per segment step of a lane
     __shared__ double sk[kCoopWaves][B][64];
     __shared__ int si[kCoopWaves][B][64];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
@@ -582,55 +581,68 @@ void lane_knn_kernel(SearchArgs a) {
         ring_bounds();
         const uint64_t* rmask = bd0.rowmask;
         const int wpr = (g[0] + 63) >> 6;
-        // Cell run of segment s (of row (y, z)) of the current ring (empty:
-        // ca > cb): row c_1 + g_1 c_2, first and last cell, clipped to the ball of squared radius rr
-        // around the query.  A ref outside that ball has key > tk (rr includes
-        // the slack tol), so it would not be admitted; cells [x_lo, x_hi] hold
-        // every ref whose x lies in [q_x - rx, q_x + rx] (same floor map as the
-        // binning).
-        auto seg_run = [&](int s, int y, int z, int& row, int& ca, int& cb) {
-            ca = 1;
-            cb = 0;
-            row = 0;
-            if (s >= nseg) return;
-            const int part = s & 1;
-            row = y + g[1] * z;
+        // Rows of the ring run y fastest: row r is (y0 + r % ny, z0 + r / ny); a row
+        // has two segments: part 0 (a full row's whole run, or the interior row's
+        // left end cell) and part 1 (the interior row's right end cell).  Runs are
+        // clipped to the ball of squared radius rr around the query: a ref outside
+        // it has key > tk (rr includes the slack tol), so it would not be
+        // admitted.  The clip is taken in cell units, q_x / h -+ rx / h, with rx / h
+        // from an FP32 square root enlarged by a relative 1e-6 and 1e-6 of a cell:
+        // that covers every rounding of the FP64 floor map of the binning, so the
+        // clip only ever widens.
+        const double qxc = (qu[0] - lo[0]) * inv_h;
+        auto row_runs = [&](int y, int z, int& a0, int& b0, int& a1, int& b1) {
             const bool full = R == 0 || z == c[2] - R || z == c[2] + R || y == c[1] - R ||
                               y == c[1] + R;
-            if (full) {
-                if (part == 0) {
-                    ca = x0;
-                    cb = x1;
-                }
-            } else {
-                const int xc = part == 0 ? c[0] - R : c[0] + R;
-                if (xc >= 0 && xc < g[0]) ca = cb = xc;
+            a0 = full ? x0 : c[0] - R;
+            b0 = full ? x1 : c[0] - R;
+            a1 = full ? 1 : c[0] + R;
+            b1 = full ? 0 : c[0] + R;
+            if (a0 < 0) {  // interior row whose left end is off the grid
+                a0 = 1;
+                b0 = 0;
             }
-            if ((ca <= cb) & (rr < __builtin_inf())) {
+            if (a1 >= g[0]) {
+                a1 = 1;
+                b1 = 0;
+            }
+            if (rr < __builtin_inf()) {
                 double s2 = 0.0;
                 if (D >= 2) {
-                    const double ylo = lo[1] + (double)y * h;
-                    const double dy = fmax(fmax(ylo - qu[1], qu[1] - (ylo + h)), 0.0);
+                    const double yl = lo[1] + (double)y * h;
+                    const double dy = fmax(fmax(yl - qu[1], qu[1] - (yl + h)), 0.0);
                     s2 = dy * dy;
                 }
                 if (D >= 3) {
-                    const double zlo = lo[2] + (double)z * h;
-                    const double dz = fmax(fmax(zlo - qu[2], qu[2] - (zlo + h)), 0.0);
+                    const double zl = lo[2] + (double)z * h;
+                    const double dz = fmax(fmax(zl - qu[2], qu[2] - (zl + h)), 0.0);
                     s2 = fma(dz, dz, s2);
                 }
                 const double rem = rr - s2;
-                if (rem < 0.0) {
-                    ca = 1;
-                    cb = 0;
-                } else {
-                    const double rx = sqrt(rem);
+                int xl = 0, xh = -1;
+                if (rem >= 0.0) {
+                    const double rc = (double)(__builtin_sqrtf((float)(rem * inv_h * inv_h)) *
+                                                   1.000001f +
+                                               1.0e-6f);
                     const double gx = (double)g[0];
-                    const double fl = fmin(fmax(floor((qu[0] - rx - lo[0]) * inv_h), -1.0), gx);
-                    const double fh = fmin(fmax(floor((qu[0] + rx - lo[0]) * inv_h), -1.0), gx);
-                    ca = max(ca, (int)fl);
-                    cb = min(cb, (int)fh);
+                    xl = (int)fmin(fmax(floor(qxc - rc), -1.0), gx);
+                    xh = (int)fmin(fmax(floor(qxc + rc), -1.0), gx);
                 }
+                a0 = max(a0, xl);
+                b0 = min(b0, xh);
+                a1 = max(a1, xl);
+                b1 = min(b1, xh);
             }
+        };
+        // does run [a, b] of row `row` hold a ref?  (runs across two occupancy
+        // words are not checked)
+        auto run_has_ref = [&](int row, int a, int b) -> bool {
+            if (a > b) return false;
+            if ((a >> 6) != (b >> 6)) return true;
+            const uint64_t m = rmask[(int64_t)row * wpr + (a >> 6)];
+            const int wdt = b - a + 1;
+            const uint64_t sel = wdt >= 64 ? ~0ull : ((1ull << wdt) - 1ull);
+            return ((m >> (a & 63)) & sel) != 0ull;
         };
         int32_t p = 0, e = 0;
         bool ring_end = false;   // ring walked: wait for the merge, then the stop test
@@ -717,57 +729,57 @@ void lane_knn_kernel(SearchArgs a) {
                 }
                 p += CH;
             } else if (sg < nseg) {
-                // The next NS segments of the ring at once: their cell runs (clipped
-                // to the admission ball) and the occupancy word of each run; the
-                // first run with a ref is loaded, the empty ones before it skipped.
-                int ca[NS], cb[NS], crow[NS];
-                uint64_t mk[NS];
-                // rows of the ring run y fastest: row r is (y0 + r % ny, z0 + r / ny)
+                // The segments of rows r0 and r0 + 1 from sg on: their runs and
+                // occupancy; the first run with a ref is loaded, the empty ones
+                // before it skipped.
                 const int r0 = sg >> 1;
-                const int rz = r0 / ny;
-                const int ry = r0 - rz * ny;
+                const int rz0 = r0 / ny;
+                const int ry0 = r0 - rz0 * ny;
+                int ra[4], rb[4], rrow[4];
+                bool has[4];
 #pragma unroll
-                for (int u = 0; u < NS; ++u) {
-                    const int dr = ((sg & 1) + u) >> 1;
-                    int yy = ry + dr, zz = rz;
-#pragma unroll
-                    for (int wrap = 0; wrap < (NS + 1) / 2; ++wrap)
-                        if (yy >= ny) {
-                            yy -= ny;
-                            ++zz;
-                        }
-                    seg_run(sg + u, y0 + yy, z0 + zz, crow[u], ca[u], cb[u]);
-                    mk[u] = ca[u] <= cb[u] ? rmask[(int64_t)crow[u] * wpr + (ca[u] >> 6)] : 0ull;
-                }
-                int pick = NS;
-#pragma unroll
-                for (int u = NS - 1; u >= 0; --u) {
-                    bool any = false;
-                    if (ca[u] <= cb[u]) {
-                        if ((ca[u] >> 6) != (cb[u] >> 6)) {
-                            any = true;  // run over several words: not checked
-                        } else {
-                            const int wdt = cb[u] - ca[u] + 1;
-                            const uint64_t sel = wdt >= 64 ? ~0ull : ((1ull << wdt) - 1ull);
-                            any = ((mk[u] >> (ca[u] & 63)) & sel) != 0ull;
-                        }
+                for (int v = 0; v < 2; ++v) {
+                    int yy = ry0 + v, zz = rz0;
+                    if (yy >= ny) {
+                        yy -= ny;
+                        ++zz;
                     }
-                    if (any) pick = u;
+                    const bool rv = 2 * (r0 + v) < nseg;
+                    const int y = y0 + yy, z = z0 + zz;
+                    int a0 = 1, b0 = 0, a1 = 1, b1 = 0;
+                    if (rv) row_runs(y, z, a0, b0, a1, b1);
+                    const int row = y + g[1] * z;
+                    if (v == 0 && (sg & 1)) {  // part 0 of row r0 already done
+                        a0 = 1;
+                        b0 = 0;
+                    }
+                    ra[2 * v] = a0;
+                    rb[2 * v] = b0;
+                    ra[2 * v + 1] = a1;
+                    rb[2 * v + 1] = b1;
+                    rrow[2 * v] = rrow[2 * v + 1] = row;
                 }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) has[u] = run_has_ref(rrow[u], ra[u], rb[u]);
+                int pick = 4;
+#pragma unroll
+                for (int u = 3; u >= 0; --u)
+                    if (has[u]) pick = u;
                 int xa = 1, xb = 0, xrow = 0;
 #pragma unroll
-                for (int u = 0; u < NS; ++u)
+                for (int u = 0; u < 4; ++u)
                     if (pick == u) {
-                        xa = ca[u];
-                        xb = cb[u];
-                        xrow = crow[u];
+                        xa = ra[u];
+                        xb = rb[u];
+                        xrow = rrow[u];
                     }
-                if (pick < NS) {
+                if (pick < 4) {
                     const int xbase = g[0] * xrow;
                     p = cs[xbase + xa];
                     e = cs[xbase + xb + 1];
                 }
-                sg = min(sg + (pick < NS ? pick + 1 : NS), nseg);
+                // next segment: after the picked one, else after row r0 + 1
+                sg = min(pick < 4 ? 2 * r0 + pick + 1 : 2 * r0 + 4, nseg);
             } else {
                 ring_end = true;
             }
