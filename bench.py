@@ -249,11 +249,11 @@ def main():
     fused_flop_pt = 2.0 * m ** 3 / 3.0 + 4.0 * m * m + E * (3 * d + 2) + E * (d + 3) + 2.0 * m * m
     fused_flop = fused_flop_pt * n
     achieved = fused_flop / (fused_avg * 1e-3) / 1e12
-    roofline = {"kernel": "vecchia_seg_kernel<double,32> fused loglik+grad (FP64)", "bound": "valu",
+    roofline = {"kernel": "vecchia_seg_kernel<double,32,3> fused loglik+grad (FP64)", "bound": "valu",
                 "achieved": achieved, "peak": PEAK_FP64_VALU_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / PEAK_FP64_VALU_TFLOPS,
                 # the committed counter summary covers the default workload only
-                "traffic": measured_traffic("vecchia_seg_kernel<double,32>") if full_size else None,
+                "traffic": measured_traffic("vecchia_seg_kernel<double,32,3>") if full_size else None,
                 "traffic_source": "profiles/loglik_road_mixture_hbm_traffic.md" if full_size else None,
                 "avg_ms": fused_avg, "algorithmic_flop": fused_flop,
                 "algorithmic_flop_per_point": fused_flop_pt}
