@@ -131,7 +131,8 @@ __device__ __forceinline__ bool row_valid(int j, uint32_t vlo, uint32_t vhi) {
     return j < S ? ((vlo >> j) & 1u) : ((vhi >> (MR - 1 - j)) & 1u);
 }
 
-__device__ __forceinline__ int tri(int i, int j) { return i * (i + 1) / 2 + j; }
+// Entry (i, j), j < i, of a strictly lower triangle stored by rows.
+__device__ __forceinline__ int stri(int i, int j) { return i * (i - 1) / 2 + j; }
 
 // Returns v unchanged but opaque to the compiler: comparisons of the lane's row
 // indices against an unrolled step index are then formed inside each step (and
@@ -146,16 +147,16 @@ __device__ __forceinline__ int opaque(int v) {
 // row's running value stops changing once its own step has passed, and z_i is
 // that value times 1 / L_ii.
 template <typename T, int MR, int NR>
-__device__ __forceinline__ void seg_fwd(const T (&L0)[MR / 2], const T (&L1)[MR], const T* dinv,
-                                        T dv0, T dv1, T (&r0)[NR], T (&r1)[NR], int lane) {
+__device__ __forceinline__ void seg_fwd(const T (&L0)[MR / 2], const T (&L1)[MR], T dv0, T dv1,
+                                        T (&r0)[NR], T (&r1)[NR], int lane) {
     constexpr int S = MR / 2;
 #pragma unroll
     for (int k = 0; k < MR; ++k) {
         const int own = k < S ? k : MR - 1 - k;
-        const T dk = dinv[k];
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
-            const T zk = seg_bcast<S>(k < S ? r0[q] : r1[q], own, lane) * dk;
+            // the owner of row k scales its running value by its 1 / L_kk
+            const T zk = seg_bcast<S>(k < S ? r0[q] * dv0 : r1[q] * dv1, own, lane);
             if (k < S) r0[q] = fma(-L0[k < S ? k : 0], zk, r0[q]);
             r1[q] = fma(-L1[k], zk, r1[q]);
         }
@@ -169,10 +170,10 @@ __device__ __forceinline__ void seg_fwd(const T (&L0)[MR / 2], const T (&L1)[MR]
 
 // L^T x = z for a lane's two rows, NR right-hand sides at once (x starts at 0:
 // rows not yet solved add nothing through the cleared entries).  z in r0 / r1 is
-// replaced by x.
+// replaced by x.  Only the owner of row k needs x_k, and it holds z_k and 1 / L_kk.
 template <typename T, int MR, int NR>
-__device__ __forceinline__ void seg_bwd(const T (&L0)[MR / 2], const T (&L1)[MR], const T* dinv,
-                                        T (&r0)[NR], T (&r1)[NR], int t, int lane) {
+__device__ __forceinline__ void seg_bwd(const T (&L0)[MR / 2], const T (&L1)[MR], T dv0, T dv1,
+                                        T (&r0)[NR], T (&r1)[NR], int t) {
     constexpr int S = MR / 2;
     T x0[NR], x1[NR];
 #pragma unroll
@@ -183,14 +184,13 @@ __device__ __forceinline__ void seg_bwd(const T (&L0)[MR / 2], const T (&L1)[MR]
 #pragma unroll
     for (int k = MR - 1; k >= 0; --k) {
         const int own = k < S ? k : MR - 1 - k;
-        const T dk = dinv[k];
         T part[NR];
 #pragma unroll
         for (int q = 0; q < NR; ++q)
             part[q] = seg_sum<S>(fma(L1[k], x1[q], k < S ? L0[k < S ? k : 0] * x0[q] : (T)0));
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
-            const T xk = (seg_bcast<S>(k < S ? r0[q] : r1[q], own, lane) - part[q]) * dk;
+            const T xk = k < S ? (r0[q] - part[q]) * dv0 : (r1[q] - part[q]) * dv1;
             if (k < S) {
                 if (t == k) x0[q] = xk;
             } else {
@@ -230,7 +230,7 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
                                          int64_t count) {
     constexpr int S = MR / 2;
     constexpr int G = 64 / S;
-    constexpr int TRI = MR * (MR + 1) / 2;
+    constexpr int TRI = MR * (MR - 1) / 2;  // strictly lower triangle
     constexpr int NE = MR + 3;  // entries per lane
     constexpr int U = 5;        // entries per step of the fill and adjoint loops
     constexpr bool REFINE = sizeof(T) == 4;
@@ -244,18 +244,16 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
     const int ny = a.ny;
     const KernParams& kp = a.kp;
 
-    // LDS per block (G blocks): FP64 covariance (lower triangle), two FP64
-    // vectors, X, the point's own target; then the factor's column broadcast and
-    // inverse diagonal (T).
+    // LDS per block (G blocks): FP64 covariance (strictly lower triangle; the
+    // diagonal is the same for every valid row), two FP64 vectors, X, the point's
+    // own target.  The factor's column broadcast (T) shares the first vector's
+    // space: it is dead before that vector is first written.
     double* Kt = reinterpret_cast<double*>(smem) + seg * TRI;
     double* xv = reinterpret_cast<double*>(smem) + G * TRI + seg * MR;
     double* vbuf = reinterpret_cast<double*>(smem) + G * (TRI + MR) + seg * MR;
     double* X = reinterpret_cast<double*>(smem) + G * (TRI + 2 * MR) + seg * (MR + 1) * DM;
     double* yself = reinterpret_cast<double*>(smem) + G * (TRI + 2 * MR + (MR + 1) * DM) + seg;
-    T* col = reinterpret_cast<T*>(reinterpret_cast<double*>(smem) +
-                                  G * (TRI + 2 * MR + (MR + 1) * DM + 1)) +
-             seg * 2 * MR;
-    T* dinv = col + MR;
+    T* col = reinterpret_cast<T*>(xv);
 
     const int64_t slot = base + seg;
     const bool pv = slot < count;
@@ -340,9 +338,7 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
                 const double Ev = (vi && vj && kp.n_ls > 0 && s.i != s.j) ? exp(-u)
                                                                          : (s.i == s.j ? 1.0 : 0.0);
                 if (e <= MR) {
-                    double K = (vi && vj) ? fma(kp.s, Ev, kp.C) : 0.0;
-                    if (s.i == s.j) K = vi ? K + diag_add : 1.0;
-                    Kt[tri(s.i, s.j)] = K;
+                    if (s.j < s.i) Kt[stri(s.i, s.j)] = (vi && vj) ? fma(kp.s, Ev, kp.C) : 0.0;
                 } else if (e == MR + 1) {
                     e0s = Ev;
                 } else {
@@ -361,15 +357,18 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
 
     // ---------------- rows into registers (type T) --------------------------
     // Entries right of a row's diagonal read neighbouring triangle entries: they
-    // are never used before being cleared below.
+    // are never used before being cleared below.  Diagonal: K_ii + w + alpha for
+    // a valid row, 1 for a padding row.
+    const double kd = fma(kp.s, 1.0, kp.C) + diag_add;
+    const double kd0 = v0 ? kd : 1.0, kd1 = v1 ? kd : 1.0;
     T L0[S], L1[MR];
     {
-        const double* r0p = Kt + tri(i0, 0);
-        const double* r1p = Kt + tri(i1, 0);
+        const double* r0p = Kt + stri(i0, 0);
+        const double* r1p = Kt + stri(i1, 0);
 #pragma unroll
-        for (int j = 0; j < S; ++j) L0[j] = (T)r0p[j];
+        for (int j = 0; j < S; ++j) L0[j] = (T)(j == i0 ? kd0 : r0p[j]);
 #pragma unroll
-        for (int j = 0; j < MR; ++j) L1[j] = (T)r1p[j];
+        for (int j = 0; j < MR; ++j) L1[j] = (T)(j == i1 ? kd1 : r1p[j]);
     }
 
     // ---------------- Cholesky (type T), no per-lane branches ---------------
@@ -378,6 +377,7 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
     // a_kk) and scale their column-k entry (the owner's becomes sqrt(a_kk)).
     // Rows above k publish and update entries that are never read.
     bool bad = false;
+    T dv0 = (T)1, dv1 = (T)1;
     double pmin = 1e300;
 #pragma unroll
     for (int k = 0; k < MR; ++k) {
@@ -388,7 +388,12 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
         bad = bad || !(dkk > (T)0);
         pmin = fmin(pmin, (double)dkk);
         const T inv = rsqrt_nr(dkk);
-        if (t == 0) dinv[k] = inv;
+        // 1 / L_kk kept by the owner of row k
+        if (k < S) {
+            if (t == k) dv0 = inv;
+        } else {
+            if (t == MR - 1 - k) dv1 = inv;
+        }
         if (k < S) {
             const T tr = L0[k] * inv * inv;
 #pragma unroll
@@ -408,7 +413,6 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
     for (int j = 0; j < S; ++j) L0[j] = j < i0 ? L0[j] : (T)0;
 #pragma unroll
     for (int j = 0; j < MR; ++j) L1[j] = j < i1 ? L1[j] : (T)0;
-    const T dv0 = dinv[i0], dv1 = dinv[i1];
 
     // FP32 form: ill-conditioned or failed blocks are redone in FP64
     const bool ill = REFINE && (bad || !(dmax / pmin <= kSegPivotRatioMax));
@@ -429,8 +433,8 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
     // x = K^-1 rhs for the lane's rows; FP32 form: one FP64-residual refinement
     auto solve = [&](double rhs0, double rhs1, double& x0, double& x1) {
         T u0[1] = {(T)rhs0}, u1[1] = {(T)rhs1};
-        seg_fwd<T, MR, 1>(L0, L1, dinv, dv0, dv1, u0, u1, lane);
-        seg_bwd<T, MR, 1>(L0, L1, dinv, u0, u1, t, lane);
+        seg_fwd<T, MR, 1>(L0, L1, dv0, dv1, u0, u1, lane);
+        seg_bwd<T, MR, 1>(L0, L1, dv0, dv1, u0, u1, t);
         x0 = (double)u0[0];
         x1 = (double)u1[0];
         if constexpr (REFINE) {
@@ -441,15 +445,15 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
 #pragma unroll 1
             for (int j = 0; j < MR; ++j) {
                 const double xj = xv[j];
-                const double K0 = j <= i0 ? Kt[tri(i0, j)] : Kt[tri(j, i0)];
-                const double K1 = j <= i1 ? Kt[tri(i1, j)] : Kt[tri(j, i1)];
+                const double K0 = j == i0 ? kd0 : (j < i0 ? Kt[stri(i0, j)] : Kt[stri(j, i0)]);
+                const double K1 = j == i1 ? kd1 : (j < i1 ? Kt[stri(i1, j)] : Kt[stri(j, i1)]);
                 q0 = fma(-K0, xj, q0);
                 q1 = fma(-K1, xj, q1);
             }
             __syncthreads();
             T w0[1] = {(T)q0}, w1[1] = {(T)q1};
-            seg_fwd<T, MR, 1>(L0, L1, dinv, dv0, dv1, w0, w1, lane);
-            seg_bwd<T, MR, 1>(L0, L1, dinv, w0, w1, t, lane);
+            seg_fwd<T, MR, 1>(L0, L1, dv0, dv1, w0, w1, lane);
+            seg_bwd<T, MR, 1>(L0, L1, dv0, dv1, w0, w1, t);
             x0 += (double)w0[0];
             x1 += (double)w1[0];
         }
@@ -476,14 +480,14 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
         // z_k = L^-1 k and z_y = L^-1 y (first column) together: sigma^2 =
         // c - z_k.z_k, mu = z_k.z_y; the gradient needs b = L^-T z_k, a = L^-T z_y
         T r0[2] = {(T)k0, (T)vbuf[i0]}, r1[2] = {(T)k1, (T)vbuf[i1]};
-        seg_fwd<T, MR, 2>(L0, L1, dinv, dv0, dv1, r0, r1, lane);
+        seg_fwd<T, MR, 2>(L0, L1, dv0, dv1, r0, r1, lane);
         const double zk0 = r0[0], zk1 = r1[0];
         s2 = c - seg_sum<S>(zk0 * zk0 + zk1 * zk1);
         sig = sqrt(s2);
         const double mu0 = seg_sum<S>(zk0 * (double)r0[1] + zk1 * (double)r1[1]);
         double av0 = 0.0, av1 = 0.0;
         if (with_grad) {
-            seg_bwd<T, MR, 2>(L0, L1, dinv, r0, r1, t, lane);
+            seg_bwd<T, MR, 2>(L0, L1, dv0, dv1, r0, r1, t);
             bv0 = r0[0];
             bv1 = r1[0];
             av0 = r0[1];
@@ -493,11 +497,11 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
         for (int o = 1; o < ny; ++o) {
             T u0[1] = {(T)(v0 ? a.y[(int64_t)j0 * ny + o] : 0.0)};
             T u1[1] = {(T)(v1 ? a.y[(int64_t)j1 * ny + o] : 0.0)};
-            seg_fwd<T, MR, 1>(L0, L1, dinv, dv0, dv1, u0, u1, lane);
+            seg_fwd<T, MR, 1>(L0, L1, dv0, dv1, u0, u1, lane);
             const double mu = seg_sum<S>(zk0 * (double)u0[0] + zk1 * (double)u1[0]);
             double a0 = 0.0, a1 = 0.0;
             if (with_grad) {
-                seg_bwd<T, MR, 1>(L0, L1, dinv, u0, u1, t, lane);
+                seg_bwd<T, MR, 1>(L0, L1, dv0, dv1, u0, u1, t);
                 a0 = u0[0];
                 a1 = u1[0];
             }
@@ -551,7 +555,7 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
                                     : mult * (Gs * bi * bj - 0.5 * (bi * aj + ati * bj));
                     W = valid ? W : 0.0;
                     const double Ev = self ? (s.first ? e0s : e1s)
-                                           : (s.j == s.i ? 1.0 : e_of(Kt[tri(s.i, jr)]));
+                                           : (s.j == s.i ? 1.0 : e_of(Kt[stri(s.i, jr)]));
                     if (!self) acc_w += s.j == s.i ? W : 0.0;
                     acc_C += W;
                     const double WE = W * Ev;
@@ -634,9 +638,9 @@ template <typename T, int MR, int DM>
 size_t seg_lds() {
     constexpr int S = MR / 2;
     constexpr int G = 64 / S;
-    constexpr int TRI = MR * (MR + 1) / 2;
+    constexpr int TRI = MR * (MR - 1) / 2;
     const size_t doubles = (size_t)G * ((size_t)(MR + 1) * DM + TRI + 2 * MR + 1);
-    return doubles * sizeof(double) + (size_t)G * 2 * MR * sizeof(T);
+    return doubles * sizeof(double);
 }
 
 template <typename T, int MR, int DM>
