@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <type_traits>
 
 #include "common.h"
 #include "vecchia.h"
@@ -232,7 +233,7 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
     constexpr int G = 64 / S;
     constexpr int TRI = MR * (MR - 1) / 2;  // strictly lower triangle
     constexpr int NE = MR + 3;  // entries per lane
-    constexpr int U = 5;        // entries per step of the fill and adjoint loops
+    constexpr int U = 5;        // entries per step of the fill loop
     constexpr bool REFINE = sizeof(T) == 4;
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x;
@@ -370,6 +371,9 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
 #pragma unroll
         for (int j = 0; j < MR; ++j) L1[j] = (T)(j == i1 ? kd1 : r1p[j]);
     }
+
+    const double* krow0 = Kt + stri(i0, 0);
+    const double* krow1 = Kt + stri(i1, 0);
 
     // ---------------- Cholesky (type T), no per-lane branches ---------------
     // Step k: the lanes publish column k of their rows, read the pivot and the
@@ -524,8 +528,10 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
     if (live && a.pt && t == 0) a.pt[p * a.pt_stride] = ll;
 
     // ---------------- adjoint contraction ----------------------------------
-    // Over the lane's entries: pairs (i, j <= i) with weight W_ij (doubled off the
-    // diagonal) against dK_ij, and the self terms (i, point) against dk_i.
+    // Over each of the lane's rows i: the self term (i, point) against dk_i and
+    // the pairs (i, j <= i) with weight W_ij (doubled off the diagonal) against
+    // dK_ij.  Column j runs over the same j in every lane (b_j, a_j and X[j] are
+    // broadcast reads).
     if (a.grad && a.pt) {
         xv[i0] = bv0;
         xv[i1] = bv1;
@@ -536,50 +542,54 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
         double accf[DM];
 #pragma unroll
         for (int f = 0; f < DM; ++f) accf[f] = 0.0;
-#pragma unroll 1
-        for (int e0 = 0; e0 < NE; e0 += U) {
+        // dk / dK weight WE against the squared distance of xi and X[j]
+        auto add_dist = [&](const double (&xi)[DM], int j, double WE) {
+            const double* xj = X + j * DM;
+            if (!ard) {
+                double u = 0.0;
 #pragma unroll
-            for (int uu = 0; uu < U; ++uu) {
-                const int e = e0 + uu;
-                if (e < NE) {
-                    const SegEntry s = seg_entry<MR>(e, i0, i1);
-                    const bool self = e > MR;  // (i, point)
-                    const int jr = self ? 0 : s.j;
-                    const bool vi = s.first ? v0 : v1;
-                    const bool valid = vi && (self || row_valid<MR>(s.j, vlo, vhi));
-                    const double bi = s.first ? bv0 : bv1;
-                    const double ati = s.first ? at0 : at1;
-                    const double bj = xv[jr], aj = vbuf[jr];
-                    const double mult = s.j == s.i ? 1.0 : 2.0;
-                    double W = self ? ati - 2.0 * Gs * bi
-                                    : mult * (Gs * bi * bj - 0.5 * (bi * aj + ati * bj));
-                    W = valid ? W : 0.0;
-                    const double Ev = self ? (s.first ? e0s : e1s)
-                                           : (s.j == s.i ? 1.0 : e_of(Kt[stri(s.i, jr)]));
-                    if (!self) acc_w += s.j == s.i ? W : 0.0;
-                    acc_C += W;
-                    const double WE = W * Ev;
-                    acc_s += WE;
-                    const double* xi = X + s.i * DM;  // (registers are saved for the factor)
-                    const double* xj = X + s.j * DM;
-                    if (!ard) {
-                        double u = 0.0;
+                for (int f = 0; f < DM; ++f) {
+                    const double tt = xi[f] - xj[f];
+                    u = fma(tt, tt, u);
+                }
+                acc_l = fma(WE * 2.0 * u, kp.h[0], acc_l);
+            } else {
 #pragma unroll
-                        for (int f = 0; f < DM; ++f) {
-                            const double tt = xi[f] - xj[f];
-                            u = fma(tt, tt, u);
-                        }
-                        acc_l = fma(WE * 2.0 * u, kp.h[0], acc_l);
-                    } else {
-#pragma unroll
-                        for (int f = 0; f < DM; ++f) {
-                            const double tt = xi[f] - xj[f];
-                            accf[f] = fma(WE * 2.0 * tt * tt, hw[f], accf[f]);
-                        }
-                    }
+                for (int f = 0; f < DM; ++f) {
+                    const double tt = xi[f] - xj[f];
+                    accf[f] = fma(WE * 2.0 * tt * tt, hw[f], accf[f]);
                 }
             }
-        }
+        };
+        auto row_adj = [&](auto ncols, int i, const double* krow, bool vi, double bi,
+                           double ati, double eis) {
+            constexpr int N = decltype(ncols)::value;
+            double xi[DM];
+#pragma unroll
+            for (int f = 0; f < DM; ++f) xi[f] = X[i * DM + f];
+            {
+                const double v = vi ? ati - 2.0 * Gs * bi : 0.0;
+                acc_C += v;
+                acc_s += v * eis;
+                add_dist(xi, MR, v * eis);
+            }
+#pragma unroll 2
+            for (int j = 0; j < N; ++j) {
+                const bool ok = vi & (j <= i) & row_valid<MR>(j, vlo, vhi);
+                const bool dg = j == i;
+                const double bj = xv[j], aj = vbuf[j];
+                double W = (dg ? 1.0 : 2.0) * (Gs * bi * bj - 0.5 * (bi * aj + ati * bj));
+                W = ok ? W : 0.0;
+                const double Ev = j < i ? e_of(krow[j]) : 1.0;
+                acc_w += dg ? W : 0.0;
+                acc_C += W;
+                const double WE = W * Ev;
+                acc_s += WE;
+                add_dist(xi, j, WE);
+            }
+        };
+        row_adj(std::integral_constant<int, S>{}, i0, krow0, v0, bv0, at0, e0s);
+        row_adj(std::integral_constant<int, MR>{}, i1, krow1, v1, bv1, at1, e1s);
         acc_w = seg_sum<S>(acc_w);
         acc_C = seg_sum<S>(acc_C);
         acc_s = seg_sum<S>(acc_s);
