@@ -254,7 +254,6 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
     double* vbuf = reinterpret_cast<double*>(smem) + G * (TRI + MR) + seg * MR;
     double* X = reinterpret_cast<double*>(smem) + G * (TRI + 2 * MR) + seg * (MR + 1) * DM;
     double* yself = reinterpret_cast<double*>(smem) + G * (TRI + 2 * MR + (MR + 1) * DM) + seg;
-    T* col = reinterpret_cast<T*>(xv);
 
     const int64_t slot = base + seg;
     const bool pv = slot < count;
@@ -279,8 +278,6 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
         X[i0 * DM + f] = xr0[f];
         X[i1 * DM + f] = xr1[f];
     }
-    vbuf[i0] = yn0;
-    vbuf[i1] = yn1;
     if (t == 0) {
 #pragma unroll
         for (int f = 0; f < DM; ++f) X[MR * DM + f] = (pv && f < d) ? a.xs[p * d + f] : 0.0;
@@ -376,19 +373,26 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
     const double* krow1 = Kt + stri(i1, 0);
 
     // ---------------- Cholesky (type T), no per-lane branches ---------------
-    // Step k: the lanes publish column k of their rows, read the pivot and the
-    // column, update their rows with the unscaled column (l_ik l_jk = a_ik a_jk /
-    // a_kk) and scale their column-k entry (the owner's becomes sqrt(a_kk)).
-    // Rows above k publish and update entries that are never read.
+    // Step k: the pivot and column k of every row are read from the published
+    // buffer; each lane updates its rows with the unscaled column (l_ik l_jk =
+    // a_ik a_jk / a_kk), column k + 1 first so that it is published into the
+    // other buffer while the rest of the update runs; then it scales its column-k
+    // entry (the owner's becomes sqrt(a_kk)).  Rows above k publish and update
+    // entries that are never read.  The two buffers share the LDS of the vectors
+    // xv and vbuf, both written only after the factorisation.
+    T* colA = reinterpret_cast<T*>(xv);
+    T* colB = reinterpret_cast<T*>(vbuf);
     bool bad = false;
     T dv0 = (T)1, dv1 = (T)1;
     double pmin = 1e300;
+    colA[i0] = L0[0];
+    colA[i1] = L1[0];
+    __syncthreads();
 #pragma unroll
     for (int k = 0; k < MR; ++k) {
-        if (k < S) col[i0] = L0[k < S ? k : 0];
-        col[i1] = L1[k];
-        __syncthreads();
-        const T dkk = col[k];
+        const T* cur = (k & 1) ? colB : colA;
+        T* nxt = (k & 1) ? colA : colB;
+        const T dkk = cur[k];
         bad = bad || !(dkk > (T)0);
         pmin = fmin(pmin, (double)dkk);
         const T inv = rsqrt_nr(dkk);
@@ -398,18 +402,25 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
         } else {
             if (t == MR - 1 - k) dv1 = inv;
         }
+        const T tr0 = k < S ? L0[k < S ? k : 0] * inv * inv : (T)0;
+        const T tr1 = L1[k] * inv * inv;
+        if (k + 1 < MR) {
+            const T cn = cur[k + 1];
+            if (k + 1 < S) {
+                L0[k + 1 < S ? k + 1 : 0] = fma(-tr0, cn, L0[k + 1 < S ? k + 1 : 0]);
+                nxt[i0] = L0[k + 1 < S ? k + 1 : 0];
+            }
+            L1[k + 1 < MR ? k + 1 : 0] = fma(-tr1, cn, L1[k + 1 < MR ? k + 1 : 0]);
+            nxt[i1] = L1[k + 1 < MR ? k + 1 : 0];
+        }
         if (k < S) {
-            const T tr = L0[k] * inv * inv;
 #pragma unroll
-            for (int j = k + 1; j < S; ++j) L0[j] = fma(-tr, col[j], L0[j]);
-            L0[k] = L0[k] * inv;
+            for (int j = k + 2; j < S; ++j) L0[j] = fma(-tr0, cur[j], L0[j]);
+            L0[k < S ? k : 0] = L0[k < S ? k : 0] * inv;
         }
-        {
-            const T tr = L1[k] * inv * inv;
 #pragma unroll
-            for (int j = k + 1; j < MR; ++j) L1[j] = fma(-tr, col[j], L1[j]);
-            L1[k] = L1[k] * inv;
-        }
+        for (int j = k + 2; j < MR; ++j) L1[j] = fma(-tr1, cur[j], L1[j]);
+        L1[k] = L1[k] * inv;
         __syncthreads();
     }
     // keep the strictly lower part only (diagonal and above cleared)
@@ -483,7 +494,7 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
     if constexpr (!REFINE) {
         // z_k = L^-1 k and z_y = L^-1 y (first column) together: sigma^2 =
         // c - z_k.z_k, mu = z_k.z_y; the gradient needs b = L^-T z_k, a = L^-T z_y
-        T r0[2] = {(T)k0, (T)vbuf[i0]}, r1[2] = {(T)k1, (T)vbuf[i1]};
+        T r0[2] = {(T)k0, (T)yn0}, r1[2] = {(T)k1, (T)yn1};
         seg_fwd<T, MR, 2>(L0, L1, dv0, dv1, r0, r1, lane);
         const double zk0 = r0[0], zk1 = r1[0];
         s2 = c - seg_sum<S>(zk0 * zk0 + zk1 * zk1);
@@ -517,8 +528,8 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
         s2 = c - seg_sum<S>(k0 * bv0 + k1 * bv1);
         sig = sqrt(s2);
         for (int o = 0; o < ny; ++o) {
-            const double y0 = o == 0 ? vbuf[i0] : (v0 ? a.y[(int64_t)j0 * ny + o] : 0.0);
-            const double y1 = o == 0 ? vbuf[i1] : (v1 ? a.y[(int64_t)j1 * ny + o] : 0.0);
+            const double y0 = o == 0 ? yn0 : (v0 ? a.y[(int64_t)j0 * ny + o] : 0.0);
+            const double y1 = o == 0 ? yn1 : (v1 ? a.y[(int64_t)j1 * ny + o] : 0.0);
             double av0, av1;
             solve(y0, y1, av0, av1);
             emit(o, seg_sum<S>(k0 * av0 + k1 * av1), av0, av1);
