@@ -728,7 +728,10 @@ void lane_knn_kernel(SearchArgs a) {
                     }
                 }
                 p += CH;
-            } else if (sg < nseg) {
+            }
+            // a lane whose run is finished finds its next run in the same step
+            if (p >= e) {
+              if (sg < nseg) {
                 // The segments of rows r0 and r0 + 1 from sg on: their runs and
                 // occupancy; the first run with a ref is loaded, the empty ones
                 // before it skipped.
@@ -780,8 +783,9 @@ void lane_knn_kernel(SearchArgs a) {
                 }
                 // next segment: after the picked one, else after row r0 + 1
                 sg = min(pick < 4 ? 2 * r0 + pick + 1 : 2 * r0 + 4, nseg);
-            } else {
+              } else {
                 ring_end = true;
+              }
             }
         }
     }
