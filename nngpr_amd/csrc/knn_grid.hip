@@ -449,6 +449,9 @@ struct SearchArgs {
     int32_t* out_idx;
 };
 
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x3 __attribute__((ext_vector_type(3)));
+
 // Sorted list of K entries, the first K - k of them fixed at (-inf, -2): the
 // last entry is then the k-th smallest admitted pair.
 template <int K>
@@ -707,19 +710,31 @@ void lane_knn_kernel(SearchArgs a) {
             if (done) continue;
             if (p < e) {
                 // the next CH refs of the segment, loaded together
-                GridPt rp[CH];
-#pragma unroll
-                for (int u = 0; u < CH; ++u) rp[u] = rpts[min(p + u, e - 1)];
+                // Only the words used are loaded (a loaded but unused register
+                // would be reused while its load is in flight, serialising the
+                // loads): u_0, u_1 as 16 bytes, u_2 and the low word of idx as 12.
+                double ru[CH][3];
+                int rix[CH];
 #pragma unroll
                 for (int u = 0; u < CH; ++u) {
-                    const double t0 = rp[u].u[0] - qu[0];
+                    const GridPt* rq = rpts + min(p + u, e - 1);
+                    const u32x4 w01 = *reinterpret_cast<const u32x4*>(rq);
+                    const u32x3 w2 = *reinterpret_cast<const u32x3*>(&rq->u[2]);
+                    ru[u][0] = __hiloint2double((int)w01.y, (int)w01.x);
+                    ru[u][1] = __hiloint2double((int)w01.w, (int)w01.z);
+                    ru[u][2] = __hiloint2double((int)w2.y, (int)w2.x);
+                    rix[u] = (int)w2.z;
+                }
+#pragma unroll
+                for (int u = 0; u < CH; ++u) {
+                    const double t0 = ru[u][0] - qu[0];
                     double key = t0 * t0;
 #pragma unroll
                     for (int f = 1; f < D; ++f) {
-                        const double tt = rp[u].u[f] - qu[f];
+                        const double tt = ru[u][f] - qu[f];
                         key = fma(tt, tt, key);
                     }
-                    const int ri = (int)rp[u].idx;
+                    const int ri = rix[u];
                     if ((p + u < e) & (ri < lim32) &
                         ((key < tk) | ((key == tk) & (ri < ti)))) {
                         bk[cnt][lane] = key;
