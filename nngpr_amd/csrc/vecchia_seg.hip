@@ -261,18 +261,31 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
 
     // ---------------- gather ---------------------------------------------
     // Coordinates of the lane's rows (registers and LDS), the point's own
-    // coordinates, and the first target column (vbuf: neighbours, yself: own).
-    const int j0 = (pv && i0 < a.m) ? a.nn[p * a.m + i0] : -1;
-    const int j1 = (pv && i1 < a.m) ? a.nn[p * a.m + i1] : -1;
+    // coordinates, and the first target column (registers; yself: own).  Every
+    // load reads a clamped, valid address and is selected afterwards, so the
+    // loads of each round are in flight together.
+    const int64_t nrow = p * a.m;
+    const int n0 = a.nn[nrow + min(i0, a.m - 1)];
+    const int n1 = a.nn[nrow + min(i1, a.m - 1)];
+    double xs_r[DM];
+#pragma unroll
+    for (int f = 0; f < DM; ++f) xs_r[f] = a.xs[p * d + min(f, d - 1)];
+    const double ys_r = a.ys ? a.ys[p * ny] : 0.0;
+    const int j0 = (pv && i0 < a.m) ? n0 : -1;
+    const int j1 = (pv && i1 < a.m) ? n1 : -1;
     const bool v0 = j0 >= 0, v1 = j1 >= 0;
+    const int64_t c0 = v0 ? j0 : 0, c1 = v1 ? j1 : 0;
     double xr0[DM], xr1[DM];
 #pragma unroll
     for (int f = 0; f < DM; ++f) {
-        xr0[f] = (v0 && f < d) ? a.x[(int64_t)j0 * d + f] : 0.0;
-        xr1[f] = (v1 && f < d) ? a.x[(int64_t)j1 * d + f] : 0.0;
+        const double a0 = a.x[c0 * d + min(f, d - 1)];
+        const double a1 = a.x[c1 * d + min(f, d - 1)];
+        xr0[f] = (v0 && f < d) ? a0 : 0.0;
+        xr1[f] = (v1 && f < d) ? a1 : 0.0;
     }
-    const double yn0 = v0 ? a.y[(int64_t)j0 * ny] : 0.0;
-    const double yn1 = v1 ? a.y[(int64_t)j1 * ny] : 0.0;
+    const double yl0 = a.y[c0 * ny], yl1 = a.y[c1 * ny];
+    const double yn0 = v0 ? yl0 : 0.0;
+    const double yn1 = v1 ? yl1 : 0.0;
 #pragma unroll
     for (int f = 0; f < DM; ++f) {
         X[i0 * DM + f] = xr0[f];
@@ -280,8 +293,8 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
     }
     if (t == 0) {
 #pragma unroll
-        for (int f = 0; f < DM; ++f) X[MR * DM + f] = (pv && f < d) ? a.xs[p * d + f] : 0.0;
-        *yself = (pv && a.ys) ? a.ys[p * ny] : 0.0;
+        for (int f = 0; f < DM; ++f) X[MR * DM + f] = (pv && f < d) ? xs_r[f] : 0.0;
+        *yself = pv ? ys_r : 0.0;
     }
     const uint64_t b0 = __ballot(v0), b1 = __ballot(v1);
     const uint32_t vlo = (uint32_t)((b0 >> (seg * S)) & ((1ull << S) - 1ull));
