@@ -64,6 +64,96 @@ def measured_traffic(kernel):
     return None
 
 
+def c3_fit(steps=50):
+    """correlated_gauss (n=16,599, d=18, White + C * RBF with 18 length scales, 20
+    theta, m=32): loglik+grad evaluations/s with the kNN recomputed (the ARD
+    length-scale ratios change the conditioning sets) and an L-BFGS-B fit of
+    `steps` iterations (FP64 factor, reference default precision)."""
+    import numpy as np
+    import scipy.optimize
+    import torch
+    from nngpr_amd import TorchNngpr, datasets
+    ds = datasets.correlated_gauss()
+    counter = {"evals": 0}
+
+    def opt(obj, theta0, bounds):
+        def f(th):
+            counter["evals"] += 1
+            return obj(th)
+        r = scipy.optimize.minimize(f, theta0, method="L-BFGS-B", jac=True, bounds=bounds,
+                                    options={"maxiter": steps})
+        return r.x, r.fun
+
+    est = TorchNngpr("cuda:0", kernel=ds["kernel"], num_nn=ds["num_nn"], optimizer=None)
+    est.fit(ds["x_train"], ds["y_train"])
+    theta = est.kernel_.theta
+    est.log_marginal_likelihood(theta, eval_gradient=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = 10
+    for r in range(reps):
+        est.log_marginal_likelihood(theta + 1e-3 * r, eval_gradient=True)
+    torch.cuda.synchronize()
+    t_eval = (time.perf_counter() - t0) / reps
+    est2 = TorchNngpr("cuda:0", kernel=ds["kernel"], num_nn=ds["num_nn"], optimizer=opt)
+    t0 = time.perf_counter()
+    est2.fit(ds["x_train"], ds["y_train"])
+    t_fit = time.perf_counter() - t0
+    return {"workload": "correlated_gauss: n=16,599 d=18 m=32, White+C*RBF(ARD, 18 length "
+                        "scales), FP64 factor",
+            "evals_per_s_knn_per_eval": 1.0 / t_eval,
+            "fit_seconds": t_fit, "fit_lbfgs_max_iter": steps,
+            "fit_objective_evals": counter["evals"],
+            "fit_loglik": float(est2.log_marginal_likelihood_value_)}
+
+
+def c4_eval(num_nn):
+    """uniform_8d (n=2,000,000, d=8): one loglik+grad evaluation in the FP32 +
+    FP64-refinement form, kNN (brute-force scans, d > 3) and fused pass timed
+    apart."""
+    import numpy as np
+    import torch
+    from nngpr_amd import _native as nat
+    from nngpr_amd import datasets, hip_ops
+    from nngpr_amd.kernel_params import get_kernel_params, get_kernel_theta_mapping
+    ds = datasets.uniform_8d(num_nn=num_nn)
+    dev = torch.device("cuda:0")
+    x = torch.from_numpy(ds["x_train"]).to(dev)
+    y = torch.from_numpy(ds["y_train"]).to(dev)
+    kern = ds["kernel"]
+    params, mapping = get_kernel_params(kern), get_kernel_theta_mapping(kern)
+    ks = nat.kernel_struct(params, mapping, len(kern.theta))
+    ls = float(np.asarray(params["length_scale"]).reshape(-1)[0])
+    F32 = hip_ops.PRECISION_FP32_REFINED
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    nn = hip_ops.knn(x, x, ls, num_nn, ref_cut=0, precision=F32)
+    torch.cuda.synchronize()
+    t_knn = time.perf_counter() - t0
+    hip_ops.vecchia_loglik(x, y, nn, ks, ds["alpha"], precision=F32, eval_gradient=True,
+                           check_pd=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        hip_ops.vecchia_loglik(x, y, nn, ks, ds["alpha"], precision=F32, eval_gradient=True,
+                               check_pd=False)
+    torch.cuda.synchronize()
+    t_fused = (time.perf_counter() - t0) / reps
+    n, d = x.shape
+    # minimum HBM bytes of the fused pass: neighbour indices, gathered coordinates
+    # and targets of every neighbour and the point, per-point outputs
+    min_bytes = n * (num_nn * 4 + (num_nn + 1) * (d + 1) * 8 + (1 + len(kern.theta)) * 8)
+    out = {"workload": "uniform_8d: n=2,000,000 d=8 m=%d, White+C*RBF(0.3), FP32 factor + "
+                       "FP64 refinement" % num_nn,
+           "knn_ms": t_knn * 1e3, "fused_loglik_grad_ms": t_fused * 1e3,
+           "evals_per_s_knn_per_eval": 1.0 / (t_knn + t_fused),
+           "fused_min_hbm_gb_per_s": min_bytes / t_fused / 1e9}
+    del x, y, nn
+    torch.cuda.empty_cache()
+    return out
+
+
 def copy_probe(nbytes):
     """Five launches of torch's copy kernel over nbytes (reads nbytes, writes nbytes)."""
     import torch
@@ -315,7 +405,9 @@ def main():
 
     serving = None
     if not args.skip_extras:
-        serving = serve_sweep(ds, (50, 200, 1000))
+        serving = serve_sweep(ds, (50, 100, 200, 500, 1000, 2000))
+        extras["correlated_gauss_d18"] = c3_fit()
+        extras["uniform_8d_2m"] = [c4_eval(32), c4_eval(64)]
 
     line = {
         "metric": "loglik+grad evals/s (reference default precision 'only-nn': FP64 factor, "

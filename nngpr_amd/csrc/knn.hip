@@ -341,7 +341,10 @@ __global__ __launch_bounds__(64) void scan_kernel(ScanArgs<T> a) {
     // the next sub-block prefetched while the current one is scored (the padded
     // layout keeps a prefetch past full_end in bounds).  The per-ref test value
     // A sub-block with no hit in any lane costs its scores and one branch.
-    constexpr int SUB = (NV * sizeof(T) <= 16) ? 8 : 4;
+    // refs per sub-block: the two ping-pong sets stay within the scalar registers
+    constexpr int SUB = (NV * sizeof(T) <= 16) ? 8
+                        : (NV * sizeof(T) <= 40) ? 4
+                        : (NV * sizeof(T) <= 72) ? 2 : 1;
     auto sub_ptr = [&](int32_t r) { return a.ref + (int64_t)(r >> 3) * (NV * 8) + (r & 7); };
     auto load_sub = [&](T (&rv)[NV][SUB], const T* bp) {
 #pragma unroll
@@ -591,7 +594,7 @@ __global__ __launch_bounds__(256) void wave_fallback_kernel(WaveFbArgs a) {
 
 // ---------------------------------------------------------------------------
 // Per-row exact fallback (one thread per row, sorted list in global memory).
-// Used for rows that failed the certificate, and for k > 64 or d > 8.
+// Used for rows that failed the certificate, and for k > 64 or d > 32.
 // ---------------------------------------------------------------------------
 struct SimpleArgs {
     const double* ru64;
@@ -668,11 +671,11 @@ __global__ void fill_i32_kernel(int32_t* p, int64_t n, int32_t v) {
 // Rows whose margin turns out too thin fail the certificate and are redone.
 inline int screen_K(int k) { return (int)round_up(k + 8, 8); }
 
-bool use_fast_scan(int d, int k) { return d >= 1 && d <= 8 && k >= 1 && k <= 64; }
+bool use_fast_scan(int d, int k) { return d >= 1 && d <= 32 && k >= 1 && k <= 64; }
 
 // Feature count the scan kernels are compiled for; extra features are zero in
 // both refs and queries, which leaves every key unchanged (t = 0 adds fma(0,0,acc)).
-inline int scan_dim(int d) { return d <= 3 ? 3 : (d == 4 ? 4 : 8); }
+inline int scan_dim(int d) { return d <= 3 ? 3 : (d == 4 ? 4 : (d <= 8 ? 8 : (d <= 16 ? 16 : 32))); }
 
 // Exact-scan list length for k neighbours.
 inline int exact_K(int k) { return k <= 16 ? 16 : (k <= 32 ? 32 : 64); }
@@ -714,6 +717,8 @@ int launch_scan(const ScanArgs<T>& a, int d, int K, hipStream_t st) {
         case 3: return launch_scan_d<T, 3, SCREEN>(a, K, st);
         case 4: return launch_scan_d<T, 4, SCREEN>(a, K, st);
         case 8: return launch_scan_d<T, 8, SCREEN>(a, K, st);
+        case 16: return launch_scan_d<T, 16, SCREEN>(a, K, st);
+        case 32: return launch_scan_d<T, 32, SCREEN>(a, K, st);
         default: return -1;
     }
 }
@@ -763,7 +768,7 @@ RefMem carve_refs(Carver& c, int64_t n_ref, int d, int precision) {
     M.mm = c.take<long long>(2 * NNGP_MAX_DIM);
     M.smax = c.take<unsigned int>(4);
     M.u64 = c.take<double>((size_t)n_ref * d + 1);
-    if (d <= 8) {
+    if (d <= 32) {
         if (precision == NNGP_PRECISION_FP32_REFINED) M.s32 = c.take<float>((size_t)rpad * (dp + 1));
         else M.s64 = c.take<double>((size_t)rpad * dp);
     }

@@ -56,7 +56,8 @@ def test_golden_small_uniform(prec, method):
 
 @pytest.mark.parametrize("prec", [0, 1])
 @pytest.mark.parametrize("cut", [None, 0, 2, -1])
-@pytest.mark.parametrize("d,ls", [(1, 0.2), (2, None), (3, "ard"), (5, 0.7), (8, "ard")])
+@pytest.mark.parametrize("d,ls", [(1, 0.2), (2, None), (3, "ard"), (5, 0.7), (8, "ard"),
+                                  (12, "ard"), (18, 0.9), (32, "ard")])
 @pytest.mark.parametrize("k", [1, 7, 33, 64])
 def test_against_oracle(prec, cut, d, ls, k, method):
     rng = np.random.default_rng(100 + d * 7 + k)
@@ -70,7 +71,7 @@ def test_against_oracle(prec, cut, d, ls, k, method):
 
 
 @pytest.mark.parametrize("prec", [0, 1])
-@pytest.mark.parametrize("d,k", [(18, 32), (3, 100), (12, 5)])
+@pytest.mark.parametrize("d,k", [(18, 32), (3, 100), (12, 5), (33, 8), (40, 70)])
 def test_per_row_path(prec, d, k):
     rng = np.random.default_rng(7)
     x = rng.normal(size=(900, d))
