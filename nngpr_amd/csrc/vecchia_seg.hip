@@ -33,7 +33,7 @@ namespace nngp {
 namespace {
 
 constexpr double kHalfLog2PiS = 0.91893853320467274178;
-constexpr int kSegMaxDim = 8;
+constexpr int kSegMaxDim = 32;
 // Pivot-ratio threshold of the FP32 form: blocks with max(diag) / min(pivot)
 // above it are re-factored in FP64.
 constexpr double kSegPivotRatioMax = 1.0e3;
@@ -60,22 +60,26 @@ constexpr int kDppMirror = 0x140;       // lane i <-> 15 - i within each 16 lane
 constexpr int kDppRowBcast = 0x150;     // + n: lane n of each 16 lanes to all of them
 
 // Sum over the segment, the same value in every lane (each level adds two equal
-// partial sums in either order).
+// partial sums in either order).  A 32-lane segment adds its two rows' sums
+// through one lane permute.
 template <int S, typename T>
 __device__ __forceinline__ T seg_sum(T v) {
-    static_assert(S == 8 || S == 16, "segments of 8 or 16 lanes");
+    static_assert(S == 8 || S == 16 || S == 32, "segments of 8, 16 or 32 lanes");
     v += dpp_x<kDppQuadSwap1>(v);
     v += dpp_x<kDppQuadSwap2>(v);
     v += dpp_x<kDppHalfMirror>(v);
-    if constexpr (S == 16) v += dpp_x<kDppMirror>(v);
+    if constexpr (S >= 16) v += dpp_x<kDppMirror>(v);
+    if constexpr (S == 32) v += __shfl_xor(v, 16, 32);
     return v;
 }
 
 // Value of lane n (0 <= n < S, a constant once unrolled) of the segment.
 template <int S, typename T>
 __device__ __forceinline__ T seg_bcast(T v, int n, int lane) {
-    static_assert(S == 8 || S == 16, "segments of 8 or 16 lanes");
-    if constexpr (S == 16) {
+    static_assert(S == 8 || S == 16 || S == 32, "segments of 8, 16 or 32 lanes");
+    if constexpr (S == 32) {
+        return __shfl(v, n, 32);
+    } else if constexpr (S == 16) {
         switch (n) {
             case 0: return dpp_x<kDppRowBcast + 0>(v);
             case 1: return dpp_x<kDppRowBcast + 1>(v);
@@ -306,8 +310,8 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
     double hw[DM];
 #pragma unroll
     for (int f = 0; f < DM; ++f) hw[f] = (ard && f < d) ? kp.h[f] : 0.0;
-    // scaled squared distance between row coordinates xi and X[j]
-    auto sqdist = [&](const double (&xi)[DM], int j) -> double {
+    // scaled squared distance between row coordinates xi (registers or LDS) and X[j]
+    auto sqdist = [&](const auto& xi, int j) -> double {
         const double* xj = X + j * DM;
         double u = 0.0;
         if (!ard) {
@@ -342,10 +346,15 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
                 const SegEntry s = seg_entry<MR>(e, i0, i1);
                 const bool vi = s.first ? v0 : v1;
                 const bool vj = s.j == MR ? true : row_valid<MR>(s.j, vlo, vhi);
-                double xi[DM];
+                double u;
+                if constexpr (DM <= 8) {
+                    double xi[DM];
 #pragma unroll
-                for (int f = 0; f < DM; ++f) xi[f] = s.first ? xr0[f] : xr1[f];
-                const double u = sqdist(xi, s.j);
+                    for (int f = 0; f < DM; ++f) xi[f] = s.first ? xr0[f] : xr1[f];
+                    u = sqdist(xi, s.j);
+                } else {  // wide rows: the lane's coordinates are read back from LDS
+                    u = sqdist(X + s.i * DM, s.j);
+                }
                 const double Ev = (vi && vj && kp.n_ls > 0 && s.i != s.j) ? exp(-u)
                                                                          : (s.i == s.j ? 1.0 : 0.0);
                 if (e <= MR) {
@@ -647,8 +656,10 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
 
 // One group of G points per block (no persistent loop: nothing loop-invariant is
 // kept live across groups).
+// (MR = 64: the LDS of a block keeps occupancy under one wave per SIMD, so the
+// register budget is a whole SIMD's.)
 template <typename T, int MR, int DM>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MR <= 32 ? 2 : 1)))
 void vecchia_seg_kernel(VecArgs a, RegExtra ex) {
     constexpr int G = 64 / (MR / 2);
     const int64_t count = ex.pcount ? (int64_t)*ex.pcount : a.npts;
@@ -697,12 +708,13 @@ void launch_seg_mr(const VecArgs& a, const RegExtra& ex, int64_t max_points, hip
 template <typename T, int MR>
 void launch_seg_d(const VecArgs& a, const RegExtra& ex, int64_t max_points, hipStream_t st) {
     if (a.d <= 3) launch_seg_mr<T, MR, 3>(a, ex, max_points, st);
+    else if (a.d <= 8) launch_seg_mr<T, MR, 8>(a, ex, max_points, st);
     else launch_seg_mr<T, MR, kSegMaxDim>(a, ex, max_points, st);
 }
 
 }  // namespace
 
-bool seg_supported(int m, int d) { return m >= 1 && m <= 32 && d >= 1 && d <= kSegMaxDim; }
+bool seg_supported(int m, int d) { return m >= 1 && m <= 64 && d >= 1 && d <= kSegMaxDim; }
 
 // fp32: FP32 factor + FP64 refinement form (flagged blocks are queued in
 // ex.flag_list); else the FP64 form.
@@ -711,10 +723,12 @@ int launch_seg(const VecArgs& a, const RegExtra& ex, bool fp32, int64_t max_poin
     if (!seg_supported(a.m, a.d)) return -1;
     if (fp32) {
         if (a.m <= 16) launch_seg_d<float, 16>(a, ex, max_points, st);
-        else launch_seg_d<float, 32>(a, ex, max_points, st);
+        else if (a.m <= 32) launch_seg_d<float, 32>(a, ex, max_points, st);
+        else launch_seg_d<float, 64>(a, ex, max_points, st);
     } else {
         if (a.m <= 16) launch_seg_d<double, 16>(a, ex, max_points, st);
-        else launch_seg_d<double, 32>(a, ex, max_points, st);
+        else if (a.m <= 32) launch_seg_d<double, 32>(a, ex, max_points, st);
+        else launch_seg_d<double, 64>(a, ex, max_points, st);
     }
     return 0;
 }

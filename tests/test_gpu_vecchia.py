@@ -85,15 +85,18 @@ def test_loglik_grad_vs_oracle(prec, m):
 
 
 @pytest.mark.parametrize("prec", [0, 1])
-def test_ard_grad_vs_oracle(prec):
-    rng = np.random.default_rng(3)
-    x = rng.normal(size=(500, 6))
+@pytest.mark.parametrize("d,m", [(6, 12), (18, 32), (32, 40)])
+def test_ard_grad_vs_oracle(prec, d, m):
+    rng = np.random.default_rng(3 + d)
+    x = rng.normal(size=(500, d))
     y = np.sin(x[:, 0]) + 0.3 * x[:, 3] + rng.normal(0, 0.05, 500)
-    ls = rng.uniform(0.8, 3.0, size=6)
-    tm = {"noise_level": 0, "rbf_level": 1, "length_scale": np.arange(2, 8)}
-    nn = vo.vecchia_knn(x, ls, 12)
-    ll_o, g_o, _ = vo.vecchia_loglik(x, y, nn, 0.02, 0.0, 0.9, ls, 1e-10, tm, 8, eval_gradient=True)
-    ll, g, _ = hip_ops.vecchia_loglik(t(x), t(y), t(nn, torch.int32), kstruct(0.02, 0.0, 0.9, ls, tm, 8),
+    ls = rng.uniform(0.8, 3.0, size=d) * np.sqrt(d / 6.0)
+    tm = {"noise_level": 0, "rbf_level": 1, "length_scale": np.arange(2, 2 + d)}
+    nn = vo.vecchia_knn(x, ls, m)
+    ll_o, g_o, _ = vo.vecchia_loglik(x, y, nn, 0.02, 0.0, 0.9, ls, 1e-10, tm, 2 + d,
+                                     eval_gradient=True)
+    ll, g, _ = hip_ops.vecchia_loglik(t(x), t(y), t(nn, torch.int32),
+                                      kstruct(0.02, 0.0, 0.9, ls, tm, 2 + d),
                                       1e-10, precision=prec, eval_gradient=True)
     assert abs(ll.item() - ll_o) / 500 <= (1e-12 if prec == 0 else 1e-6)
     np.testing.assert_allclose(g.cpu().numpy(), g_o, rtol=1e-9 if prec == 0 else 1e-5, atol=1e-8)
