@@ -576,6 +576,7 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
 #pragma unroll
         for (int f = 0; f < DM; ++f) accf[f] = 0.0;
         // dk / dK weight WE against the squared distance of xi and X[j]
+        // (accumulated without the factor 2 h_f, applied at the end)
         auto add_dist = [&](const double (&xi)[DM], int j, double WE) {
             const double* xj = X + j * DM;
             if (!ard) {
@@ -585,15 +586,18 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
                     const double tt = xi[f] - xj[f];
                     u = fma(tt, tt, u);
                 }
-                acc_l = fma(WE * 2.0 * u, kp.h[0], acc_l);
+                acc_l = fma(WE, u, acc_l);
             } else {
 #pragma unroll
                 for (int f = 0; f < DM; ++f) {
                     const double tt = xi[f] - xj[f];
-                    accf[f] = fma(WE * 2.0 * tt * tt, hw[f], accf[f]);
+                    accf[f] = fma(WE * tt, tt, accf[f]);
                 }
             }
         };
+        // Row i: the self term, the diagonal (W_ii = G_s b_i^2 - b_i a_i, with
+        // dK_ii free of the length scales) and the off-diagonal pairs j < i with
+        // W_ij doubled: 2 G_s b_i b_j - b_i a_j - a_i b_j = c1 b_j - b_i a_j.
         auto row_adj = [&](auto ncols, int i, const double* krow, bool vi, double bi,
                            double ati, double eis) {
             constexpr int N = decltype(ncols)::value;
@@ -605,24 +609,29 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
                 acc_C += v;
                 acc_s += v * eis;
                 add_dist(xi, MR, v * eis);
+                const double wd = vi ? (Gs * bi - ati) * bi : 0.0;
+                acc_w += wd;
+                acc_C += wd;
+                acc_s += wd;
             }
+            const double c1 = 2.0 * Gs * bi - ati;
 #pragma unroll 2
             for (int j = 0; j < N; ++j) {
-                const bool ok = vi & (j <= i) & row_valid<MR>(j, vlo, vhi);
-                const bool dg = j == i;
+                const bool ok = vi & (j < i) & row_valid<MR>(j, vlo, vhi);
                 const double bj = xv[j], aj = vbuf[j];
-                double W = (dg ? 1.0 : 2.0) * (Gs * bi * bj - 0.5 * (bi * aj + ati * bj));
-                W = ok ? W : 0.0;
-                const double Ev = j < i ? e_of(krow[j]) : 1.0;
-                acc_w += dg ? W : 0.0;
+                const double W = ok ? fma(c1, bj, -bi * aj) : 0.0;
+                // entries right of the diagonal are other rows' data: never used
+                const double WE = ok ? W * e_of(krow[j]) : 0.0;
                 acc_C += W;
-                const double WE = W * Ev;
                 acc_s += WE;
                 add_dist(xi, j, WE);
             }
         };
         row_adj(std::integral_constant<int, S>{}, i0, krow0, v0, bv0, at0, e0s);
         row_adj(std::integral_constant<int, MR>{}, i1, krow1, v1, bv1, at1, e1s);
+        acc_l *= 2.0 * kp.h[0];
+#pragma unroll
+        for (int f = 0; f < DM; ++f) accf[f] *= 2.0 * hw[f];
         acc_w = seg_sum<S>(acc_w);
         acc_C = seg_sum<S>(acc_C);
         acc_s = seg_sum<S>(acc_s);
