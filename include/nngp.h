@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define NNGP_ABI_VERSION 1
+#define NNGP_ABI_VERSION 2
 #define NNGP_MAX_DIM 64   /* maximum number of input features */
 #define NNGP_MAX_NN 256   /* maximum neighbours per point */
 
@@ -43,13 +43,30 @@ typedef enum {
 } nngp_precision;
 
 /*
- * Decomposed covariance  k(x_i, x_j) = s * exp(-sum_f (x_if - x_jf)^2 / (2 l_f^2))
- *                                      + C + w * [i == j (by position)]
- * i.e. WhiteKernel(w) + ConstantKernel(C) + ConstantKernel(s) * RBF(l), the form
- * the reference reduces a kernel tree to (cuda_kernel_accelerators.py:520-590).
- * theta_* give the index of each parameter's log in kernel.theta, or -1 when the
- * parameter is fixed or absent (cuda_kernel_accelerators.py:592-649); ARD length
- * scales occupy theta_length_scale .. theta_length_scale + n_length_scale - 1.
+ * Stationary part of the covariance as a function of the distance in length-scale
+ * units r = sqrt(sum_f (x_if - x_jf)^2 / l_f^2):
+ *   RBF         exp(-r^2 / 2)
+ *   Matern 1/2  exp(-r)
+ *   Matern 3/2  (1 + a) exp(-a),             a = sqrt(3) r
+ *   Matern 5/2  (1 + a + a^2 / 3) exp(-a),   a = sqrt(5) r
+ * (batched_kernels.py:844-920 RBF, :1003-1131 Matern; Matern with nu = inf is the
+ * RBF).
+ */
+typedef enum {
+    NNGP_COV_RBF = 0,
+    NNGP_COV_MATERN_12 = 1,
+    NNGP_COV_MATERN_32 = 2,
+    NNGP_COV_MATERN_52 = 3
+} nngp_covariance;
+
+/*
+ * Decomposed covariance  k(x_i, x_j) = s * E(r_ij) + C + w * [i == j (by position)]
+ * with E the stationary part above, i.e. WhiteKernel(w) + ConstantKernel(C) +
+ * ConstantKernel(s) * RBF(l) or Matern(l, nu), the form the reference reduces a
+ * kernel tree to (cuda_kernel_accelerators.py:520-590).  theta_* give the index of
+ * each parameter's log in kernel.theta, or -1 when the parameter is fixed or
+ * absent (cuda_kernel_accelerators.py:592-649); ARD length scales occupy
+ * theta_length_scale .. theta_length_scale + n_length_scale - 1.
  */
 typedef struct {
     double noise_level;     /* w */
@@ -62,6 +79,7 @@ typedef struct {
     int32_t theta_length_scale;
     int32_t n_theta;        /* length of kernel.theta */
     double length_scale[NNGP_MAX_DIM];
+    int32_t covariance;     /* nngp_covariance of the stationary part */
 } nngp_kernel;
 
 /* Human-readable text of a status code (static storage). */

@@ -39,7 +39,26 @@ class NngpKernel(ctypes.Structure):
         ("theta_length_scale", ctypes.c_int32),
         ("n_theta", ctypes.c_int32),
         ("length_scale", ctypes.c_double * NNGP_MAX_DIM),
+        ("covariance", ctypes.c_int32),
     ]
+
+
+# nngp_covariance: stationary part of the covariance
+COV_RBF = 0
+COV_MATERN_12 = 1
+COV_MATERN_32 = 2
+COV_MATERN_52 = 3
+_MATERN_FAMILY = {0.5: COV_MATERN_12, 1.5: COV_MATERN_32, 2.5: COV_MATERN_52}
+
+
+def covariance_family(nu):
+    """nngp_covariance of a Matern smoothness nu (None or inf: the RBF)."""
+    if nu is None or np.isinf(nu):
+        return COV_RBF
+    fam = _MATERN_FAMILY.get(float(nu))
+    if fam is None:
+        raise NotImplementedError("Matern nu=%r is not accelerated (0.5, 1.5, 2.5 or inf)" % nu)
+    return fam
 
 
 class NngpStats(ctypes.Structure):
@@ -170,6 +189,7 @@ def kernel_struct(params, mapping, n_theta):
         for i, v in enumerate(lsa):
             k.length_scale[i] = float(v)
         k.rbf_level = float(params.get("rbf_level", 1.0))
+        k.covariance = covariance_family(params.get("matern_nu"))
 
     def idx(name):
         v = mapping.get(name)

@@ -1,7 +1,8 @@
 """Batched GP kernels over any numpy-like array module (numpy, torch on a device).
 
 The in-scope family of the reference's batched kernels (batched_kernels.py:10-297,
-545-920): WhiteKernel, ConstantKernel, RBF (isotropic or ARD), Sum, Product.  Each
+545-1131): WhiteKernel, ConstantKernel, RBF and Matern (isotropic or ARD), Sum,
+Product.  Each
 class subclasses its scikit-learn twin, so hyperparameters, theta (log
 parameters), bounds and cloning behave exactly like scikit-learn; evaluation takes
 the array module first and accepts inputs of shape (*batch, n, d):
@@ -210,3 +211,57 @@ class RBF(BatchedKernel, kernels.RBF):
 
     def diag(self, xp, X):
         return xp.ones(X.shape[:-1], dtype=X.dtype)
+
+
+class Matern(RBF, kernels.Matern):
+    """Matern kernel for nu in {0.5, 1.5, 2.5, inf} (batched_kernels.py:923-1131);
+    any other nu raises NotImplementedError, as in the reference (:1081, :1120).
+
+    With r the distance in length-scale units and a = sqrt(2 nu) r:
+    nu = 0.5: exp(-r); 1.5: (1 + a) exp(-a); 2.5: (1 + a + a^2/3) exp(-a);
+    inf: exp(-r^2 / 2).  d K / d log l_f = G * (dx_f / l_f)^2 with G = K / r (0 at
+    r = 0), 3 exp(-a), 5/3 (1 + a) exp(-a) and K respectively.
+    """
+
+    def __call__(self, xp, X, Y=None, eval_gradient=False):
+        ls = xp.asarray(_check_length_scale(X, self.length_scale), dtype=X.dtype)
+        Xs = X / ls
+        if Y is None:
+            diff = Xs[..., :, None, :] - Xs[..., None, :, :]
+        else:
+            if eval_gradient:
+                raise ValueError("Gradient can only be evaluated when Y is None.")
+            diff = Xs[..., :, None, :] - (Y / ls)[..., None, :, :]
+        sq = diff * diff
+        d2 = sq.sum(-1)
+        nu = self.nu
+        if nu == 0.5:
+            r = xp.sqrt(d2)
+            K = xp.exp(-r)
+        elif nu == 1.5:
+            a = xp.sqrt(3.0 * d2)
+            K = (1.0 + a) * xp.exp(-a)
+        elif nu == 2.5:
+            a = xp.sqrt(5.0 * d2)
+            K = (1.0 + a + a * a / 3.0) * xp.exp(-a)
+        elif np.isinf(nu):
+            K = xp.exp(-0.5 * d2)
+        else:
+            raise NotImplementedError("Matern nu=%r is supported only for 0.5, 1.5, 2.5 and inf"
+                                      % nu)
+        if Y is not None or not eval_gradient:
+            return K
+        if self.hyperparameter_length_scale.fixed:
+            return K, xp.empty((*X.shape[:-1], X.shape[-2], 0), dtype=X.dtype)
+        if nu == 0.5:
+            G = xp.where(r > 0, K / xp.where(r > 0, r, xp.ones_like(r)), xp.zeros_like(r))
+        elif nu == 1.5:
+            G = 3.0 * xp.exp(-a)
+        elif nu == 2.5:
+            G = 5.0 / 3.0 * (1.0 + a) * xp.exp(-a)
+        else:
+            G = K
+        if not self.anisotropic or np.ndim(_check_length_scale(X, self.length_scale)) == 0 \
+                or np.size(self.length_scale) == 1:
+            return K, (G * d2)[..., None]
+        return K, G[..., None] * sq

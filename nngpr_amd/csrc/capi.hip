@@ -42,7 +42,7 @@ void timing_begin(int id, hipStream_t st) {
     if (!t.on) return;
     hipEvent_t e;
     if (hipEventCreate(&e) != hipSuccess) return;
-    hipEventRecord(e, st);
+    (void)hipEventRecord(e, st);
     t.pending[id] = e;
 }
 
@@ -52,7 +52,7 @@ void timing_end(int id, hipStream_t st) {
     if (!t.on || !t.pending[id]) return;
     hipEvent_t e;
     if (hipEventCreate(&e) != hipSuccess) return;
-    hipEventRecord(e, st);
+    (void)hipEventRecord(e, st);
     t.ev[id].push_back({t.pending[id], e});
     t.pending[id] = nullptr;
 }
@@ -79,6 +79,9 @@ int to_kern_params(const nngp_kernel* k, int d, KernParams* out) {
     kp.t_s = k->n_length_scale > 0 ? k->theta_rbf_level : -1;
     kp.t_l = k->n_length_scale > 0 ? k->theta_length_scale : -1;
     kp.n_theta = k->n_theta;
+    if (k->covariance < NNGP_COV_RBF || k->covariance > NNGP_COV_MATERN_52)
+        return NNGP_ERR_INVALID_ARGUMENT;
+    kp.fam = k->covariance;
     for (int f = 0; f < kp.n_ls; ++f) {
         const double l = k->length_scale[f];
         if (!(l > 0.0)) return NNGP_ERR_INVALID_ARGUMENT;
@@ -248,8 +251,8 @@ int nngp_timing_read(int kernel, int reset, double* total_ms, int64_t* launches)
     *launches = (int64_t)t.ev[kernel].size();
     if (reset) {
         for (auto& pr : t.ev[kernel]) {
-            hipEventDestroy(pr.first);
-            hipEventDestroy(pr.second);
+            (void)hipEventDestroy(pr.first);
+            (void)hipEventDestroy(pr.second);
         }
         t.ev[kernel].clear();
     }
@@ -426,7 +429,7 @@ int nngp_model_create(const double* x_train, const double* y_train, int64_t n_tr
     M->x = cm.take<double>((size_t)n_train * d);
     M->y = cm.take<double>((size_t)n_train * ny);
     char* rmem = cm.take<char>(rb);
-    hipGetDevice(&M->device);
+    (void)hipGetDevice(&M->device);
     M->n = n_train;
     M->d = d;
     M->ny = ny;
@@ -439,14 +442,14 @@ int nngp_model_create(const double* x_train, const double* y_train, int64_t n_tr
     hipError_t e2 = hipMemcpyAsync(M->y, y_train, sizeof(double) * n_train * ny,
                                    hipMemcpyDeviceToDevice, st);
     if (e1 != hipSuccess || e2 != hipSuccess) {
-        hipFree(M->mem);
+        (void)hipFree(M->mem);
         delete M;
         return NNGP_ERR_HIP;
     }
     rc = knn_prepare_refs(M->x, n_train, d, scale, n_scale, precision, rmem, rb, &M->refs, st);
     if (rc == NNGP_OK && hipStreamSynchronize(st) != hipSuccess) rc = NNGP_ERR_HIP;
     if (rc) {
-        hipFree(M->mem);
+        (void)hipFree(M->mem);
         delete M;
         return rc;
     }

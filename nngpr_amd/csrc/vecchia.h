@@ -10,8 +10,45 @@ struct KernParams {
     int n_ls;                   // 0, 1 or d
     int t_w, t_C, t_s, t_l;     // theta indices or -1
     int n_theta;
+    int fam;                    // nngp_covariance of the stationary part
     double h[NNGP_MAX_DIM];     // 1 / (2 l_f^2)
 };
+
+// Stationary part as a function of u = sum_f h_f dx_f^2 = r^2 / 2 (r: distance in
+// length-scale units): E(u), and the factor G with dE/dlog l_f = G * 2 h_f dx_f^2
+// (isotropic: dE/dlog l = G * 2 u).
+//   RBF         E = exp(-u)                                G = E
+//   Matern 1/2  E = exp(-r)                                G = E / r   (0 at r = 0)
+//   Matern 3/2  E = (1 + a) exp(-a),           a = sqrt(3) r   G = 3 exp(-a)
+//   Matern 5/2  E = (1 + a + a^2/3) exp(-a),   a = sqrt(5) r   G = 5/3 (1 + a) exp(-a)
+// (dE/dr = -E, -a^2 exp(-a) / r and -a^2 (1 + a) exp(-a) / (3 r) respectively, and
+// dr/dlog l_f = -2 h_f dx_f^2 / r.)  The r = 0 value of Matern 1/2 follows the
+// reference, which divides by 1 where the distance is 0 (batched_kernels.py:1101-1104).
+__device__ __forceinline__ double cov_e(int fam, double u) {
+    if (fam == NNGP_COV_RBF) return exp(-u);
+    if (fam == NNGP_COV_MATERN_12) return exp(-sqrt(2.0 * u));
+    if (fam == NNGP_COV_MATERN_32) {
+        const double a = sqrt(6.0 * u);
+        return (1.0 + a) * exp(-a);
+    }
+    const double a = sqrt(10.0 * u);
+    return fma(a, fma(a, 1.0 / 3.0, 1.0), 1.0) * exp(-a);
+}
+
+// G from u and E = cov_e(fam, u) (E may carry the rounding of a stored entry).
+__device__ __forceinline__ double cov_g(int fam, double u, double E) {
+    if (fam == NNGP_COV_RBF) return E;
+    if (fam == NNGP_COV_MATERN_12) {
+        const double r = sqrt(2.0 * u);
+        return r > 0.0 ? E / r : 0.0;
+    }
+    if (fam == NNGP_COV_MATERN_32) {
+        const double a = sqrt(6.0 * u);
+        return 3.0 * E / (1.0 + a);
+    }
+    const double a = sqrt(10.0 * u);
+    return (5.0 / 3.0) * (1.0 + a) * E / fma(a, fma(a, 1.0 / 3.0, 1.0), 1.0);
+}
 
 // One fused pass over points: loglik (ys != NULL) or predict (ys == NULL).
 struct VecArgs {

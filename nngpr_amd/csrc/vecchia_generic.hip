@@ -18,8 +18,8 @@ namespace nngp {
 
 namespace {
 
-// k(a, b) without the white term; E (the RBF exponential) and u_f (the per-feature
-// exponent terms, ARD) or u (isotropic) are returned for gradient use.
+// k(a, b) without the white term; E (the stationary part) and u = sum_f h_f dx_f^2
+// are returned for gradient use.
 __device__ __forceinline__ double cov_entry(const KernParams& kp, const double* xa,
                                             const double* xb, int d, double* E_out,
                                             double* u_out) {
@@ -41,7 +41,7 @@ __device__ __forceinline__ double cov_entry(const KernParams& kp, const double* 
             u = fma(t * t, kp.h[f], u);
         }
     }
-    double E = exp(-u);
+    double E = cov_e(kp.fam, u);
     *E_out = E;
     *u_out = u;
     return fma(kp.s, E, kp.C);
@@ -176,11 +176,12 @@ __global__ __launch_bounds__(256) void vecchia_generic_kernel(VecArgs a, int64_t
         cov_entry(kp, xsp, xj, d, &E, &u);
         acc_C += v;
         acc_s += v * E;
-        if (kp.n_ls == 1) acc_l += v * E * 2.0 * u;
+        double Gd = cov_g(kp.fam, u, E);
+        if (kp.n_ls == 1) acc_l += v * Gd * 2.0 * u;
         if (ard)
             for (int f = 0; f < d; ++f) {
                 double tt = xsp[f] - xj[f];
-                acc_lf[f] += v * E * 2.0 * tt * tt * kp.h[f];
+                acc_lf[f] += v * Gd * 2.0 * tt * tt * kp.h[f];
             }
         // neighbour - neighbour entries (l <= j; off-diagonal counted twice)
         for (int l = 0; l <= j; ++l) {
@@ -194,11 +195,12 @@ __global__ __launch_bounds__(256) void vecchia_generic_kernel(VecArgs a, int64_t
             if (l == j) acc_w += W;
             acc_C += W;
             acc_s += W * E;
-            if (kp.n_ls == 1) acc_l += W * E * 2.0 * u;
+            Gd = cov_g(kp.fam, u, E);
+            if (kp.n_ls == 1) acc_l += W * Gd * 2.0 * u;
             if (ard)
                 for (int f = 0; f < d; ++f) {
                     double tt = xj[f] - xl[f];
-                    acc_lf[f] += W * E * 2.0 * tt * tt * kp.h[f];
+                    acc_lf[f] += W * Gd * 2.0 * tt * tt * kp.h[f];
                 }
         }
     }
@@ -248,13 +250,14 @@ __global__ __launch_bounds__(256) void fill_kernel(const double* x, int d, const
         if (kp.t_w >= 0 && i == j) g[kp.t_w] += kp.w;
         if (kp.t_C >= 0) g[kp.t_C] += kp.C;
         if (kp.t_s >= 0) g[kp.t_s] += kp.s * E;
-        if (kp.t_l >= 0 && kp.n_ls == 1) g[kp.t_l] += kp.s * E * 2.0 * u;
+        const double Gd = kp.n_ls > 0 ? cov_g(kp.fam, u, E) : 0.0;
+        if (kp.t_l >= 0 && kp.n_ls == 1) g[kp.t_l] += kp.s * Gd * 2.0 * u;
         if (kp.t_l >= 0 && kp.n_ls > 1) {
             const double* xa = x + (int64_t)ji * d;
             const double* xb = x + (int64_t)jj * d;
             for (int f = 0; f < d; ++f) {
                 double tt = xa[f] - xb[f];
-                g[kp.t_l + f] += kp.s * E * 2.0 * tt * tt * kp.h[f];
+                g[kp.t_l + f] += kp.s * Gd * 2.0 * tt * tt * kp.h[f];
             }
         }
     }

@@ -6,7 +6,7 @@
 //
 //   fill     the (MR+1)(MR+2)/2 distinct covariance entries (neighbours + the
 //            point itself) are computed once in FP64, spread over the group's
-//            lanes, and kept in LDS as the RBF exponential E_ij;
+//            lanes, and kept in LDS as the stationary part E_ij;
 //   factor   each lane loads its row into registers (type T) and the block is
 //            factored right-looking with one LDS column broadcast per step;
 //   solves   forward substitution with register rows and lane shuffles; back
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(64) void vecchia_reg_kernel(VecArgs a, RegExtra ex)
             MR == 64 ? vball : ((vball >> (g * MR)) & ((1ULL << (MR & 63)) - 1ULL));
         const bool vr = jr >= 0;
 
-        // ---------------- fill (FP64 exponentials) ---------------------------
+        // ---------------- fill (FP64 stationary part) ------------------------
         constexpr int TRI = MR * (MR + 1) / 2;
         for (int e = r; e < TRI + MR; e += MR) {
             int i, j;
@@ -166,7 +166,7 @@ __global__ __launch_bounds__(64) void vecchia_reg_kernel(VecArgs a, RegExtra ex)
                         u = fma(t * t, kp.h[f], u);
                     }
                 }
-                E = exp(-u);
+                E = cov_e(kp.fam, u);
             }
             if (self) {
                 E0g[i] = E;
@@ -313,18 +313,37 @@ __global__ __launch_bounds__(64) void vecchia_reg_kernel(VecArgs a, RegExtra ex)
             double acc_w = 0.0, acc_C = 0.0, acc_s = 0.0, acc_l = 0.0;
             const double* xr = Xg + r * d;
             const double* xs = Xg + MR * d;
+            // Matern with ARD: row r's stored E (read by this lane only from here
+            // on) is replaced by the length-scale factor G for the per-feature pass.
+            const bool g_pass = kp.n_ls > 1 && kp.fam != NNGP_COV_RBF;
+            // G of the pair (row r, xo) from its E, and u into *uo
+            auto pair_g = [&](const double* xo, double E, double* uo) -> double {
+                double u = 0.0;
+                if (kp.n_ls == 1) {
+                    for (int f = 0; f < d; ++f) {
+                        double t = xr[f] - xo[f];
+                        u = fma(t, t, u);
+                    }
+                    u *= kp.h[0];
+                } else {
+                    for (int f = 0; f < d; ++f) {
+                        double t = xr[f] - xo[f];
+                        u = fma(t * t, kp.h[f], u);
+                    }
+                }
+                *uo = u;
+                return cov_g(kp.fam, u, E);
+            };
             if (vr) {
                 const double v = at - 2.0 * Gs * bvec;
                 const double E0 = E0g[r];
                 acc_C += v;
                 acc_s += v * E0;
-                if (kp.n_ls == 1) {
-                    double u = 0.0;
-                    for (int f = 0; f < d; ++f) {
-                        double t = xr[f] - xs[f];
-                        u = fma(t, t, u);
-                    }
-                    acc_l += v * E0 * 2.0 * u * kp.h[0];
+                if (kp.n_ls == 1 || g_pass) {
+                    double u;
+                    const double G0 = pair_g(xs, E0, &u);
+                    if (kp.n_ls == 1) acc_l += v * G0 * 2.0 * u;
+                    else E0g[r] = G0;
                 }
                 for (int l = 0; l < MR; ++l) {
                     if (!((vbits >> l) & 1ULL)) continue;
@@ -334,14 +353,11 @@ __global__ __launch_bounds__(64) void vecchia_reg_kernel(VecArgs a, RegExtra ex)
                     if (l == r) acc_w += W;
                     acc_C += W;
                     acc_s += W * E;
-                    if (kp.n_ls == 1) {
-                        const double* xl = Xg + l * d;
-                        double u = 0.0;
-                        for (int f = 0; f < d; ++f) {
-                            double t = xr[f] - xl[f];
-                            u = fma(t, t, u);
-                        }
-                        acc_l += W * E * 2.0 * u * kp.h[0];
+                    if (kp.n_ls == 1 || g_pass) {
+                        double u;
+                        const double Gl = pair_g(Xg + l * d, E, &u);
+                        if (kp.n_ls == 1) acc_l += W * Gl * 2.0 * u;
+                        else Eg[r * (MR + 1) + l] = Gl;
                     }
                 }
             }

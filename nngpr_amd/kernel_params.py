@@ -1,11 +1,14 @@
-"""Reduction of a kernel tree to the accelerated form  White + C + s * RBF.
+"""Reduction of a kernel tree to the accelerated form  White + C + s * E,  with E
+an RBF or a Matern (nu = 1/2, 3/2, 5/2) stationary part.
 
 Restates the reference's RbfAccelerator.get_kernel_params and
 get_kernel_theta_mapping (cuda_kernel_accelerators.py:520-649) and the
 CudaKernelAccelerator cache (:18-164), with one deliberate difference: Matern is a
 subclass of RBF in scikit-learn, and the reference's subclass test lets it through
-as if it were an RBF; here it is rejected (NotImplementedError) like any other
-non-accelerated kernel.
+as if it were an RBF (its accelerated fill then computes RBF entries).  Here a
+Matern with nu in {0.5, 1.5, 2.5} keeps its own covariance (the closed forms of
+batched_kernels.py:1003-1131, the reference's generic path), nu = inf is the RBF,
+and any other nu raises NotImplementedError like every non-accelerated kernel.
 """
 import copy
 import numbers
@@ -15,14 +18,24 @@ from sklearn.gaussian_process import kernels
 
 
 def get_kernel_params(kernel):
-    """Dict of {noise_level, constant_value, rbf_level, length_scale} describing
-    the kernel as  noise*[i==j] + constant + rbf_level*RBF(length_scale).
+    """Dict of {noise_level, constant_value, rbf_level, length_scale[, matern_nu]}
+    describing the kernel as  noise*[i==j] + constant + rbf_level*E(length_scale),
+    E the RBF or, with matern_nu, the Matern of that nu.
 
     Raises NotImplementedError for kernels outside that family
     (cuda_kernel_accelerators.py:534-590).
     """
     if isinstance(kernel, kernels.Matern):
-        raise NotImplementedError("Matern kernels are not accelerated")
+        nu = float(kernel.nu)
+        if not (np.isinf(nu) or nu in (0.5, 1.5, 2.5)):
+            raise NotImplementedError("Matern nu=%r is not accelerated (0.5, 1.5, 2.5 or inf)"
+                                      % kernel.nu)
+        ls = kernel.length_scale
+        out = {"length_scale": ls if isinstance(ls, numbers.Number)
+               else np.array(ls).reshape((1, -1))}
+        if not np.isinf(nu):
+            out["matern_nu"] = nu
+        return out
     if isinstance(kernel, kernels.WhiteKernel):
         return {"noise_level": kernel.noise_level}
     if isinstance(kernel, kernels.ConstantKernel):
@@ -105,7 +118,8 @@ def get_kernel_theta_mapping(kernel):
             raise NotImplementedError("Unsupported case of non-adjacent length-scale thetas")
         return ind
 
-    mapping = {p: find(p) for p in params}
+    # matern_nu is a fixed property of the kernel, not a hyperparameter
+    mapping = {p: find(p) for p in params if p != "matern_nu"}
     mapping = {k: v for k, v in mapping.items() if v is not None}
     n_found = sum(len(v) if isinstance(v, np.ndarray) else 1 for v in mapping.values())
     if n_found != n_theta:

@@ -11,6 +11,8 @@ Kernel parameters are passed explicitly as the decomposition of
     constant    C   (ConstantKernel summed with the RBF part)
     level       s   (ConstantKernel multiplying the RBF part)
     length      ls  (None: no RBF part; scalar: isotropic; (d,) array: ARD)
+    nu              None or inf: RBF stationary part; 0.5, 1.5, 2.5: Matern of
+                    that nu (batched_kernels.py:1003-1131)
 
 and ``theta_map`` maps each of 'noise_level', 'constant_value', 'rbf_level',
 'length_scale' to its index (or index array for ARD) in kernel.theta
@@ -96,14 +98,47 @@ def knn_lexsort(ref, query, ls, k, ref_cut=None):
     return out
 
 
-def fill(xg, valid, w, C, s, ls, theta_map=None, n_theta=0, eval_gradient=False):
+def stationary(u, nu=None):
+    """Stationary part E and its length-scale factor G at u = r^2 / 2, r the
+    distance in length-scale units, with dE/dlog ls_f = G dx_f^2 / ls_f^2.
+
+    RBF (nu None or inf; batched_kernels.py:844-920): E = exp(-u), G = E.
+    Matern (batched_kernels.py:1069-1124), a = sqrt(2 nu) r:
+      nu = 0.5  E = exp(-r)                  G = E / r (0 where r = 0, :1101-1104)
+      nu = 1.5  E = (1 + a) exp(-a)          G = 3 exp(-a)
+      nu = 2.5  E = (1 + a + a^2/3) exp(-a)  G = 5/3 (1 + a) exp(-a)
+    """
+    u = np.asarray(u, dtype=np.float64)
+    if nu is None or np.isinf(nu):
+        E = np.exp(-u)
+        return E, E
+    r = np.sqrt(2.0 * u)
+    if nu == 0.5:
+        E = np.exp(-r)
+        G = np.where(r > 0, E / np.where(r > 0, r, 1.0), 0.0)
+    elif nu == 1.5:
+        a = np.sqrt(3.0) * r
+        E = (1.0 + a) * np.exp(-a)
+        G = 3.0 * np.exp(-a)
+    elif nu == 2.5:
+        a = np.sqrt(5.0) * r
+        E = (1.0 + a + a * a / 3.0) * np.exp(-a)
+        G = 5.0 / 3.0 * (1.0 + a) * np.exp(-a)
+    else:
+        raise NotImplementedError("Matern nu=%r" % nu)
+    return E, G
+
+
+def fill(xg, valid, w, C, s, ls, theta_map=None, n_theta=0, eval_gradient=False, nu=None):
     """Neighbour covariance blocks and their theta gradients.
 
     Restates the accelerated fill (cuda_kernel_accelerators.py:778-861):
-    K_jl = s * exp(-sum_f dx_f^2 / (2 ls_f^2)) + C + w * [j == l] for valid j, l
-    (white noise by position, :831); rows/cols of invalid slots are the identity
-    with zero gradient (:787-811).  Gradients w.r.t. log-parameters (:833-857):
-    d/dlog w = w [j == l], d/dlog C = C, d/dlog s = s E, d/dlog ls_f = s E dx_f^2/ls_f^2.
+    K_jl = s * E(u_jl) + C + w * [j == l] for valid j, l, u = sum_f dx_f^2 / (2 ls_f^2)
+    and E the stationary part (`stationary`; the reference's accelerated fill has
+    the RBF only) (white noise by position, :831); rows/cols of invalid slots are
+    the identity with zero gradient (:787-811).  Gradients w.r.t. log-parameters
+    (:833-857): d/dlog w = w [j == l], d/dlog C = C, d/dlog s = s E,
+    d/dlog ls_f = s G dx_f^2/ls_f^2.
 
     xg: (B, M, d) gathered coordinates; valid: (B, M) bool.
     Returns K (B, M, M) and, if eval_gradient, Kg (B, M, M, n_theta).
@@ -114,11 +149,12 @@ def fill(xg, valid, w, C, s, ls, theta_map=None, n_theta=0, eval_gradient=False)
     sq = diff * diff
     if ls is None:
         E = np.zeros((B, M, M))
+        Gf = E
         half = np.zeros_like(sq)
     else:
         lsa = np.asarray(ls, dtype=np.float64).reshape(-1)
         half = sq / (2.0 * lsa * lsa) if lsa.size == d else sq / (2.0 * lsa[0] * lsa[0])
-        E = np.exp(-half.sum(-1))
+        E, Gf = stationary(half.sum(-1), nu)
     eye = np.eye(M, dtype=bool)[None]
     pair = valid[:, :, None] & valid[:, None, :]
     K = s * E + C + w * eye
@@ -136,10 +172,10 @@ def fill(xg, valid, w, C, s, ls, theta_map=None, n_theta=0, eval_gradient=False)
     if tm.get("length_scale") is not None and ls is not None:
         idx = np.asarray(tm["length_scale"]).reshape(-1)
         if idx.size == 1 and np.asarray(ls).size == 1:
-            Kg[..., idx[0]] = np.where(pair, s * E * 2.0 * half.sum(-1), 0.0)
+            Kg[..., idx[0]] = np.where(pair, s * Gf * 2.0 * half.sum(-1), 0.0)
         else:
             for f, t in enumerate(idx):
-                Kg[..., t] = np.where(pair, s * E * 2.0 * half[..., f], 0.0)
+                Kg[..., t] = np.where(pair, s * Gf * 2.0 * half[..., f], 0.0)
     return K, Kg
 
 
@@ -153,7 +189,7 @@ def _cho_solve(L, b):
 
 
 def vecchia_loglik(x, y, nn, w, C, s, ls, alpha, theta_map=None, n_theta=0,
-                   eval_gradient=False, rows=None):
+                   eval_gradient=False, rows=None, nu=None):
     """Vecchia log-likelihood of the training set (and its theta gradient).
 
     Restates Nngpr._log_likelihood_worker (base_nngpr.py:530-614): for row i with
@@ -177,7 +213,7 @@ def vecchia_loglik(x, y, nn, w, C, s, ls, alpha, theta_map=None, n_theta=0,
     idx = np.concatenate([rows[:, None], nn[rows]], axis=1)
     valid = idx > -1
     xg = x[np.where(valid, idx, 0)]
-    K, Kg = fill(xg, valid, w, C, s, ls, theta_map, n_theta, eval_gradient)
+    K, Kg = fill(xg, valid, w, C, s, ls, theta_map, n_theta, eval_gradient, nu)
     B, M = idx.shape
     K = K + alpha * (valid[:, :, None] & np.eye(M, dtype=bool)[None])
     L = _chol(K[:, 1:, 1:])
@@ -207,7 +243,7 @@ def vecchia_loglik(x, y, nn, w, C, s, ls, alpha, theta_map=None, n_theta=0,
     return ll_rows.sum(), g.sum(axis=(1, 2)), ll_rows
 
 
-def vecchia_adjoint_grad(x, y, nn, w, C, s, ls, alpha, theta_map, n_theta, rows=None):
+def vecchia_adjoint_grad(x, y, nn, w, C, s, ls, alpha, theta_map, n_theta, rows=None, nu=None):
     """Closed-form adjoint of the Vecchia log-likelihood (no K_gradient solves).
 
     With Kt = K_NN + alpha I, k = K_N,self, c = K_self,self + alpha, a = Kt^-1 y_N,
@@ -228,7 +264,7 @@ def vecchia_adjoint_grad(x, y, nn, w, C, s, ls, alpha, theta_map, n_theta, rows=
     idx = np.concatenate([rows[:, None], np.asarray(nn)[rows]], axis=1)
     valid = idx > -1
     xg = x[np.where(valid, idx, 0)]
-    K, Kg = fill(xg, valid, w, C, s, ls, theta_map, n_theta, True)
+    K, Kg = fill(xg, valid, w, C, s, ls, theta_map, n_theta, True, nu)
     B, M = idx.shape
     K = K + alpha * (valid[:, :, None] & np.eye(M, dtype=bool)[None])
     Kt = K[:, 1:, 1:]
@@ -253,7 +289,7 @@ def vecchia_adjoint_grad(x, y, nn, w, C, s, ls, alpha, theta_map, n_theta, rows=
     return g.sum(0)
 
 
-def predict(x_train, y_train, xq, nn, w, C, s, ls, alpha):
+def predict(x_train, y_train, xq, nn, w, C, s, ls, alpha, nu=None):
     """Only-train prediction (mean, std) of query points.
 
     Restates Nngpr._predict_worker (base_nngpr.py:1101-1127) with the neighbour
@@ -275,7 +311,7 @@ def predict(x_train, y_train, xq, nn, w, C, s, ls, alpha):
     idx = np.concatenate([np.arange(nt, nt + nq)[:, None], nn], axis=1)
     valid = idx > -1
     xg = x_full[np.where(valid, idx, 0)]
-    K, _ = fill(xg, valid, w, C, s, ls)
+    K, _ = fill(xg, valid, w, C, s, ls, nu=nu)
     M = idx.shape[1]
     train = (idx >= 0) & (idx < nt)
     K = K + alpha * (train[:, :, None] & np.eye(M, dtype=bool)[None])

@@ -4,6 +4,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 from nngpr_amd import _native as nat
@@ -30,15 +31,48 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_status_strings():
     lib = nat.lib()
-    assert lib.nngp_abi_version() == 1
+    assert lib.nngp_abi_version() == 2
     assert nat.status_text(0) == "success"
     assert "positive definite" in nat.status_text(nat.NNGP_ERR_NOT_POSITIVE_DEFINITE)
 
 
 def test_kernel_struct_layout():
-    # double x3, int32 x6, pad to 8, double[64]
-    assert ctypes.sizeof(nat.NngpKernel) == 3 * 8 + 6 * 4 + 64 * 8
+    # double x3, int32 x6, double[64], int32 covariance, pad to 8
+    assert ctypes.sizeof(nat.NngpKernel) == 3 * 8 + 6 * 4 + 64 * 8 + 8
     assert nat.NngpKernel.length_scale.offset == 48
+    assert nat.NngpKernel.covariance.offset == 48 + 64 * 8
+
+
+@pytest.mark.parametrize("nu,fam", [(0.5, nat.COV_MATERN_12), (1.5, nat.COV_MATERN_32),
+                                    (2.5, nat.COV_MATERN_52), (np.inf, nat.COV_RBF)])
+@pytest.mark.parametrize("ard", [False, True])
+def test_matern_decomposition(nu, fam, ard):
+    """C * Matern + White reduces like C * RBF + White, with the family kept aside
+    (nu is not a hyperparameter: it maps to no theta)."""
+    from sklearn.gaussian_process import kernels as skk
+    from nngpr_amd import batched_kernels as bk
+    from nngpr_amd.kernel_params import get_kernel_params, get_kernel_theta_mapping
+    ls = np.array([0.3, 0.7]) if ard else 0.4
+    for mod in (skk, bk):
+        k = mod.ConstantKernel(1.3) * mod.Matern(ls, nu=nu) + mod.WhiteKernel(0.2)
+        p, mp = get_kernel_params(k), get_kernel_theta_mapping(k)
+        assert ("matern_nu" in p) == (not np.isinf(nu))
+        assert "matern_nu" not in mp
+        assert mp["rbf_level"] == 0 and mp["noise_level"] == (3 if ard else 2)
+        np.testing.assert_array_equal(np.atleast_1d(mp["length_scale"]),
+                                      [1, 2] if ard else [1])
+        ks = nat.kernel_struct(p, mp, len(k.theta))
+        assert ks.covariance == fam and ks.n_length_scale == (2 if ard else 1)
+        assert ks.rbf_level == pytest.approx(1.3) and ks.noise_level == pytest.approx(0.2)
+
+
+def test_matern_other_nu_not_accelerated():
+    from sklearn.gaussian_process import kernels as skk
+    from nngpr_amd.kernel_params import can_accelerate_kernel
+    assert not can_accelerate_kernel(skk.Matern(0.4, nu=1.0) + skk.WhiteKernel(0.1))
+    assert can_accelerate_kernel(skk.Matern(0.4, nu=2.5) + skk.WhiteKernel(0.1))
+    # two stationary parts cannot be summed
+    assert not can_accelerate_kernel(skk.Matern(0.4, nu=0.5) + skk.RBF(0.2))
 
 
 def test_workspace_queries_are_host_only():

@@ -3,7 +3,8 @@
 Each oracle function is checked against something it does not share code with:
   knn             numpy lexsort of (key, index) on exact-key / well-separated inputs;
                   the reference's own ref_cut cases (None, 0, 2, -1)
-  fill            scikit-learn's kernels on the gathered points (values and gradients)
+  fill            scikit-learn's kernels on the gathered points (values and gradients),
+                  RBF and Matern nu = 1/2, 3/2, 5/2 (iso and ARD, duplicate points)
   vecchia_loglik  scikit-learn's exact GaussianProcessRegressor when m >= n - 1
                   (the reference's own known-answer test, nngpr_base_tester.py:174-192);
                   dense conditional Gaussians by brute force when m < n;
@@ -121,6 +122,94 @@ def test_fill_ard_matches_sklearn():
         Kr, Gr = k(xg[b], eval_gradient=True)
         np.testing.assert_allclose(K[b], Kr, rtol=1e-13)
         np.testing.assert_allclose(G[b], Gr, rtol=1e-12, atol=1e-15)
+
+
+MATERN_NU = [0.5, 1.5, 2.5]
+
+
+def matern_case(nu, ard, d=3):
+    ls = np.linspace(0.4, 1.1, d) if ard else 0.45
+    k = skk.ConstantKernel(1.3) * skk.Matern(ls, nu=nu) + skk.WhiteKernel(0.2)
+    tm = {"rbf_level": 0, "length_scale": np.arange(1, d + 1) if ard else 1,
+          "noise_level": d + 1 if ard else 2}
+    return k, ls, tm, (d + 2 if ard else 3)
+
+
+@pytest.mark.parametrize("nu", MATERN_NU)
+@pytest.mark.parametrize("ard", [False, True])
+def test_fill_matern_matches_sklearn(nu, ard):
+    """Matern values and log-parameter gradients against scikit-learn's Matern,
+    with a duplicated point (r = 0: the nu = 1/2 derivative is 0 there)."""
+    rng = np.random.default_rng(21)
+    xg = rng.random((4, 6, 3))
+    xg[1, 4] = xg[1, 2]
+    k, ls, tm, nt = matern_case(nu, ard)
+    K, G = vo.fill(xg, np.ones((4, 6), bool), 0.2, 0.0, 1.3, ls, tm, nt, True, nu=nu)
+    for b in range(4):
+        Kr, Gr = k(xg[b], eval_gradient=True)
+        np.testing.assert_allclose(K[b], Kr, rtol=1e-13, atol=1e-15)
+        np.testing.assert_allclose(G[b], Gr, rtol=1e-12, atol=1e-15)
+
+
+def test_fill_matern_planted_rbf_derivative_fails(monkeypatch):
+    """Taking the RBF's length-scale factor (G = E) for a Matern is caught."""
+    rng = np.random.default_rng(22)
+    xg = rng.random((2, 5, 3))
+    k, ls, tm, nt = matern_case(1.5, True)
+    real = vo.stationary
+    monkeypatch.setattr(vo, "stationary", lambda u, nu=None: (real(u, nu)[0],) * 2)
+    _, G = vo.fill(xg, np.ones((2, 5), bool), 0.2, 0.0, 1.3, ls, tm, nt, True, nu=1.5)
+    _, Gr = k(xg[0], eval_gradient=True)
+    assert not np.allclose(G[0], Gr, rtol=1e-6)
+
+
+@pytest.mark.parametrize("nu", MATERN_NU)
+def test_exact_gp_matern_loglik_and_grad(nu):
+    x, _, y = ref_test_data()
+    y = y[:, :2]
+    k, ls, tm, nt = matern_case(nu, True, d=4)
+    gpr = skgp.GaussianProcessRegressor(kernel=k, alpha=1e-10)
+    gpr.X_train_, gpr.y_train_, gpr.kernel_ = x, y, k
+    ll_ref, g_ref = gpr.log_marginal_likelihood(k.theta, eval_gradient=True)
+    nn = vo.vecchia_knn(x, ls, x.shape[0] - 1)
+    ll, g, _ = vo.vecchia_loglik(x, y, nn, 0.2, 0.0, 1.3, ls, 1e-10, tm, nt, eval_gradient=True,
+                                 nu=nu)
+    assert abs(ll / ll_ref - 1) < 1e-10
+    np.testing.assert_allclose(g, g_ref, rtol=1e-8, atol=1e-8)
+    g_adj = vo.vecchia_adjoint_grad(x, y, nn, 0.2, 0.0, 1.3, ls, 1e-10, tm, nt, nu=nu)
+    np.testing.assert_allclose(g_adj, g_ref, rtol=1e-8, atol=1e-8)
+
+
+@pytest.mark.parametrize("nu", MATERN_NU)
+def test_exact_gp_matern_predict(nu):
+    x, x_eval, y = ref_test_data()
+    k = skk.ConstantKernel(0.8) * skk.Matern(0.6, nu=nu) + skk.WhiteKernel(0.3)
+    gpr = skgp.GaussianProcessRegressor(kernel=k, optimizer=None, alpha=1e-10).fit(x, y[:, 1])
+    mean_ref, std_ref = gpr.predict(x_eval, return_std=True)
+    nn = vo.knn(x, x_eval, 0.6, x.shape[0])
+    mean, std = vo.predict(x, y[:, 1], x_eval, nn, 0.3, 0.0, 0.8, 0.6, 1e-10, nu=nu)
+    np.testing.assert_allclose(mean[:, 0], mean_ref, atol=1e-9)
+    np.testing.assert_allclose(std, std_ref, atol=1e-9)
+
+
+@pytest.mark.parametrize("nu", MATERN_NU)
+def test_vecchia_matern_grad_central_differences(nu):
+    rng = np.random.default_rng(23)
+    x = rng.random((70, 2))
+    y = np.sin(5 * x[:, 0]) + rng.normal(0, 0.1, 70)
+    theta = np.log([0.9, 0.3, 0.02])
+    nn = vo.vecchia_knn(x, np.exp(theta[1]), 6)
+
+    def f(t):
+        c, l, w = np.exp(t)
+        return vo.vecchia_loglik(x, y, nn, w, 0.0, c, l, 1e-8, nu=nu)[0]
+
+    c, l, w = np.exp(theta)
+    _, g, _ = vo.vecchia_loglik(x, y, nn, w, 0.0, c, l, 1e-8, THETA_MAP, 3, eval_gradient=True,
+                                nu=nu)
+    h = 1e-5
+    fd = np.array([(f(theta + h * e) - f(theta - h * e)) / (2 * h) for e in np.eye(3)])
+    np.testing.assert_allclose(g, fd, rtol=1e-6, atol=1e-6)
 
 
 # ------------------------------------------------------------ exact GP known answers
