@@ -180,6 +180,60 @@ int nngp_predict(const double* x_train, const double* y_train, int64_t n_train, 
                  double* out_std, void* workspace, size_t workspace_bytes, void* stream);
 
 /*
+ * Sequential conditioning: prediction where query i is conditioned on training rows
+ * and on earlier queries (conditioning_method 'full' / 'full-double-nn') and the
+ * joint predictive covariance (return_cov).  Replaces the sequential branch of
+ * Nngpr.predict (base_nngpr.py:896-1030).
+ *   x_train (n_train x d), y_train (n_train x ny): FP64 device arrays (NULL when
+ *     n_train == 0); x_query (n_query x d).
+ *   nn (n_query x m int32, device): row i indexes the concatenation
+ *     [x_train; x_query]: j < n_train a training row, j = n_train + q an earlier
+ *     query q < i, -1 an empty slot (anything else: NNGP_ERR_INVALID_ARGUMENT).
+ *   alpha is added to the variance of training neighbours only (:948-950).
+ *   output NNGP_COND_MEAN: out_mean (n_query x ny) only.
+ *          NNGP_COND_STD: out_mean and out_std (n_query), the standard deviation
+ *            through the reference's sparse partial covariance (:918-1008): the
+ *            covariance of two query neighbours is the one stored when the later
+ *            had the earlier among its neighbours, else 0.
+ *          NNGP_COND_COV: out_mean, out_cov (n_query x n_query, row-major, the
+ *            joint covariance of the dense recursion, :933-1001) and out_std
+ *            (its diagonal's square root, or NULL).
+ * Outputs are for the normalised targets (the caller undoes normalize_y).  The
+ * call reads nn on the host (it synchronises `stream`) to order the queries by
+ * dependency level; every query of a level is computed at once, one level after
+ * the other.  FP64 only, as the reference's sequential branch.
+ */
+typedef enum {
+    NNGP_COND_MEAN = 0,
+    NNGP_COND_STD = 1,
+    NNGP_COND_COV = 2
+} nngp_cond_output;
+size_t nngp_predict_sequential_workspace_bytes(int64_t n_train, int64_t n_query, int m, int d,
+                                               int ny, int output);
+int nngp_predict_sequential(const double* x_train, const double* y_train, int64_t n_train,
+                            const double* x_query, int64_t n_query, int d, int ny,
+                            const int32_t* nn, int m, const nngp_kernel* kernel, double alpha,
+                            int output, double* out_mean, double* out_std, double* out_cov,
+                            void* workspace, size_t workspace_bytes, void* stream);
+
+/*
+ * Samples of the sequential process.  Replaces the sequential branch of
+ * Nngpr.sample_y (base_nngpr.py:680-752): y_i = K_xn K^-1 y_N + sigma_i z_i, with
+ * y_N the training targets and the samples already drawn at earlier query
+ * neighbours (sample by sample), sigma_i = sqrt(K_00 - K_xn K^-1 K_nx) (0 where
+ * that is not positive).  Same inputs as nngp_predict_sequential; normals
+ * (n_query x ny x n_samples, device) are the caller's standard normal draws z and
+ * out_samples has the same shape.  Synchronises `stream`.
+ */
+size_t nngp_sample_sequential_workspace_bytes(int64_t n_train, int64_t n_query, int m, int d,
+                                              int ny);
+int nngp_sample_sequential(const double* x_train, const double* y_train, int64_t n_train,
+                           const double* x_query, int64_t n_query, int d, int ny,
+                           const int32_t* nn, int m, const nngp_kernel* kernel, double alpha,
+                           int n_samples, const double* normals, double* out_samples,
+                           void* workspace, size_t workspace_bytes, void* stream);
+
+/*
  * Resident prediction model for serving.  No single reference counterpart: it
  * keeps the state the reference rebuilds on every predict call
  * (_find_nn_indices_for_train_and_eval + _predict_worker, base_nngpr.py:1370-1464,

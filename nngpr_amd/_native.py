@@ -25,6 +25,11 @@ NNGP_ERR_KNN_OVERFLOW = -6
 PRECISION_FP64 = 0
 PRECISION_FP32_REFINED = 1
 
+# nngp_cond_output
+COND_MEAN = 0
+COND_STD = 1
+COND_COV = 2
+
 
 class NngpKernel(ctypes.Structure):
     """Mirror of `nngp_kernel` (include/nngp.h)."""
@@ -85,6 +90,13 @@ SIGNATURES = [
     ("nngp_predict_workspace_bytes", _SZ, [_I64, _I, _I, _I, _I]),
     ("nngp_predict", _I, [_P, _P, _I64, _I, _I, _P, _I64, _P, _I,
                           ctypes.POINTER(NngpKernel), _D, _I, _P, _P, _P, _SZ, _P]),
+    ("nngp_predict_sequential_workspace_bytes", _SZ, [_I64, _I64, _I, _I, _I, _I]),
+    ("nngp_predict_sequential", _I, [_P, _P, _I64, _P, _I64, _I, _I, _P, _I,
+                                     ctypes.POINTER(NngpKernel), _D, _I, _P, _P, _P, _P, _SZ,
+                                     _P]),
+    ("nngp_sample_sequential_workspace_bytes", _SZ, [_I64, _I64, _I, _I, _I]),
+    ("nngp_sample_sequential", _I, [_P, _P, _I64, _P, _I64, _I, _I, _P, _I,
+                                    ctypes.POINTER(NngpKernel), _D, _I, _P, _P, _P, _SZ, _P]),
     ("nngp_read_stats", _I, [_P, ctypes.POINTER(NngpStats), _P]),
     ("nngp_model_create", _I, [_P, _P, _I64, _I, _I, ctypes.POINTER(NngpKernel), _D, _I, _P, _I, _I,
                                ctypes.POINTER(ctypes.c_void_p), _P]),

@@ -8,9 +8,10 @@ prediction is one kNN launch sequence plus one fused fill/factor/solve launch
 sequence on one MI355X (nngpr_amd.hip_ops), instead of per-batch tensor algebra
 distributed over worker processes.
 
-Out of scope here (raise NotImplementedError): sequential conditioning
-(conditioning_method 'full' / 'full-double-nn'), return_cov, sample_y, and kernels
-outside White + Constant * RBF.
+Sequential conditioning (conditioning_method 'full' / 'full-double-nn'),
+return_cov and sample_y run on the HIP engine too (nngp_predict_sequential /
+nngp_sample_sequential).  Kernels outside White + Constant * (RBF or Matern
+nu = 1/2, 3/2, 5/2) raise NotImplementedError.
 """
 import multiprocessing
 from numbers import Integral
@@ -335,9 +336,7 @@ class Nngpr(GaussianProcessRegressor):
             dtype, ensure_2d = None, False
         X = validate_data(self, X, ensure_2d=ensure_2d, dtype=dtype, reset=False)
         if return_cov or conditioning_method != "only-train":
-            raise NotImplementedError(
-                "Sequential conditioning ('full', 'full-double-nn') and return_cov are not "
-                "implemented on the HIP engine; use conditioning_method='only-train'.")
+            return self._predict_sequential(X, return_std, return_cov, conditioning_method)
         worker = self._worker
         xp = worker.get_array_module()
         if not hasattr(self, "X_train_"):
@@ -375,5 +374,134 @@ class Nngpr(GaussianProcessRegressor):
             std = std[:, 0]
         return mean, std
 
+    # ------------------------------------------------- sequential conditioning
+    def _conditioning_nn(self, kernel, x_train, xq, method, knn_prec):
+        """Neighbour lists of the queries over [x_train; xq]
+        (_find_nn_indices_for_train_and_eval, base_nngpr.py:1370-1464, without the
+        leading self column): 'only-train' the nearest training rows; 'full' the
+        nearest of the training rows and the earlier queries; 'full-double-nn' the
+        nearest training rows, then the nearest earlier queries, valid entries first."""
+        nt, nq = x_train.shape[0], xq.shape[0]
+        dev = xq.device
+        scale = self._worker.knn_scale(kernel, self.nn_type)
+        empty = torch.empty((nq, 0), dtype=torch.int32, device=dev)
+        if method == "only-train":
+            k = min(nt, self.num_nn)
+            return hip_ops.knn(x_train, xq, scale, k, precision=knn_prec) if k > 0 else empty
+        if method == "full":
+            k = min(self.num_nn, nt + nq)
+            x_full = torch.cat([x_train, xq], dim=0)
+            return hip_ops.knn(x_full, xq, scale, k, ref_cut=nt, precision=knn_prec)
+        kq = min(self.num_nn, nq - 1)
+        qn = hip_ops.knn(xq, xq, scale, kq, ref_cut=0, precision=knn_prec) if kq > 0 else empty
+        qn = torch.where(qn >= 0, qn + nt, qn)
+        if nt == 0:
+            return qn
+        kt = min(self.num_nn, nt)
+        tr = hip_ops.knn(x_train, xq, scale, kt, precision=knn_prec)
+        # valid training entries, then valid query entries, then -1 (the extra last
+        # column takes the scatter of the empty query slots)
+        merged = torch.full((nq, kt + kq + 1), -1, dtype=torch.int32, device=dev)
+        merged[:, :kt] = tr
+        if kq > 0:
+            pos = (tr >= 0).sum(dim=1, keepdim=True) + torch.arange(kq, device=dev)[None, :]
+            pos = torch.where(qn >= 0, pos, torch.full_like(pos, kt + kq))
+            merged.scatter_(1, pos, qn)
+        return merged[:, :kt + kq].contiguous()
+
+    def _sequential_setup(self, X, method):
+        worker = self._worker
+        xp = worker.get_array_module()
+        xq = xp.from_numpy(np.ascontiguousarray(X, dtype=np.float64))
+        if hasattr(self, "X_train_"):
+            kernel = self.kernel_
+            x_dev, y_dev, _ = self._device_train()
+        else:
+            kernel = self.kernel if self.kernel is not None else self.get_default_kernel()
+            x_dev = torch.empty((0, xq.shape[1]), dtype=torch.float64, device=xq.device)
+            y_dev = torch.empty((0, 1), dtype=torch.float64, device=xq.device)
+        knn_prec, _ = _precisions(self.allow_downcast_f32)
+        ks, _ = worker.kernel_struct(kernel)
+        nn = self._conditioning_nn(kernel, x_dev, xq, method, knn_prec)
+        return x_dev, y_dev, xq, nn, ks
+
+    def _y_scale(self, dtype):
+        if not hasattr(self, "_y_train_std"):
+            return None, None
+        return (np.asarray(self._y_train_std).astype(dtype).reshape((1, -1)),
+                np.asarray(self._y_train_mean).astype(dtype).reshape((1, -1)))
+
+    def _predict_sequential(self, X, return_std, return_cov, method):
+        """Sequential branch of predict (base_nngpr.py:896-1030) on the HIP engine:
+        mean, the sparse-partial-covariance std, or the joint covariance."""
+        x_dev, y_dev, xq, nn, ks = self._sequential_setup(X, method)
+        output = hip_ops.COND_COV if return_cov else (hip_ops.COND_STD if return_std
+                                                      else hip_ops.COND_MEAN)
+        mean_d, std_d, cov_d = hip_ops.predict_sequential(x_dev, y_dev, xq, nn, ks,
+                                                          float(self.alpha), output)
+        mean = mean_d.cpu().numpy()
+        ystd, ymean = self._y_scale(mean.dtype)
+        if ystd is not None:
+            mean = ystd * mean + ymean
+        if mean.shape[1] == 1:
+            mean = mean[:, 0]
+        if return_std:
+            std = std_d.cpu().numpy().reshape((-1, 1))
+            if ystd is not None:
+                std = std * ystd
+            return mean, (std[:, 0] if std.shape[1] == 1 else std)
+        if return_cov:
+            cov = cov_d.cpu().numpy()
+            if ystd is not None:
+                cov = cov[:, :, None] * (ystd.reshape((1, 1, -1)) ** 2)
+                if cov.shape[2] == 1:
+                    cov = cov[:, :, 0]
+            return mean, cov
+        return mean
+
+    def _generator(self, random_state, dev):
+        """torch generator on the device seeded from random_state (None, int or a
+        numpy RandomState, as check_random_state accepts)."""
+        g = torch.Generator(device=dev)
+        if random_state is None:
+            g.seed()
+        elif isinstance(random_state, (Integral, np.integer)):
+            g.manual_seed(int(random_state))
+        else:
+            g.manual_seed(int(check_random_state(random_state).randint(0, 2 ** 31 - 1)))
+        return g
+
     def sample_y(self, X, n_samples=1, random_state=0, conditioning_method=None):
-        raise NotImplementedError("sample_y is not implemented on the HIP engine")
+        """Samples of the process at X (base_nngpr.py:616-761): 'only-train' draws
+        independent normals around the only-train mean/std; the sequential methods
+        draw each query given the samples already drawn at its earlier query
+        neighbours.  Random draws come from a torch generator on the device seeded
+        by random_state; the draws differ from the reference's generators."""
+        conditioning_method = "only-train" if conditioning_method is None else conditioning_method
+        if conditioning_method not in {"only-train", "full", "full-double-nn"}:
+            raise ValueError("conditioning_method must be 'only-train', 'full' or 'full-double-nn'")
+        if self.kernel is None or self.kernel.requires_vector_input:
+            dtype, ensure_2d = "numeric", True
+        else:
+            dtype, ensure_2d = None, False
+        X = validate_data(self, X, ensure_2d=ensure_2d, dtype=dtype, reset=False)
+        nq = X.shape[0]
+        dev = self._worker.get_array_module().torch_device
+        gen = self._generator(random_state, dev)
+        if conditioning_method == "only-train":
+            mu, sigma = self.predict(X, return_std=True)
+            mu = np.asarray(mu).reshape((nq, -1))
+            sigma = np.asarray(sigma).reshape((nq, -1))
+            z = torch.randn((nq, mu.shape[1], n_samples), generator=gen, dtype=torch.float64,
+                            device=dev).cpu().numpy()
+            y = mu[:, :, None] + sigma[:, :, None] * z
+            return y[:, 0, :] if y.shape[1] == 1 else y
+        x_dev, y_dev, xq, nn, ks = self._sequential_setup(X, conditioning_method)
+        ny = y_dev.shape[1]
+        z = torch.randn((nq, ny, n_samples), generator=gen, dtype=torch.float64, device=dev)
+        y = hip_ops.sample_sequential(x_dev, y_dev, xq, nn, ks, float(self.alpha), z)
+        y = y.cpu().numpy()
+        ystd, ymean = self._y_scale(y.dtype)
+        if ystd is not None:
+            y = y * ystd[:, :, None] + ymean[:, :, None]
+        return y[:, 0, :] if ny == 1 else y

@@ -9,6 +9,7 @@
 
 #include "common.h"
 #include "knn.h"
+#include "conditioning.h"
 #include "vecchia.h"
 #include "vecchia_reg.h"
 
@@ -398,6 +399,96 @@ int nngp_predict(const double* x_train, const double* y_train, int64_t n_train, 
     if (rc) return rc;
     NNGP_HIP_CHECK(hipGetLastError());
     return NNGP_OK;
+}
+
+size_t nngp_predict_sequential_workspace_bytes(int64_t n_train, int64_t n_query, int m, int d,
+                                               int ny, int output) {
+    if (n_train < 0 || n_query < 0 || m < 1 || d < 1 || ny < 1) return 0;
+    return cond_workspace_bytes(n_train, n_query, m, d, ny, output);
+}
+
+namespace {
+
+int check_sequential(const double* x_train, const double* y_train, int64_t n_train,
+                     const double* x_query, int64_t n_query, int d, int ny, const int32_t* nn,
+                     int m, const nngp_kernel* kernel, KernParams* kp) {
+    if (d < 1 || d > NNGP_MAX_DIM || ny < 1 || m < 1 || m > NNGP_MAX_NN || n_train < 0 ||
+        n_query < 0 || n_train + n_query > INT32_MAX)
+        return NNGP_ERR_INVALID_ARGUMENT;
+    if (n_train > 0 && (!x_train || !y_train)) return NNGP_ERR_INVALID_ARGUMENT;
+    if (n_query > 0 && (!x_query || !nn)) return NNGP_ERR_INVALID_ARGUMENT;
+    return to_kern_params(kernel, d, kp);
+}
+
+}  // namespace
+
+int nngp_predict_sequential(const double* x_train, const double* y_train, int64_t n_train,
+                            const double* x_query, int64_t n_query, int d, int ny,
+                            const int32_t* nn, int m, const nngp_kernel* kernel, double alpha,
+                            int output, double* out_mean, double* out_std, double* out_cov,
+                            void* workspace, size_t workspace_bytes, void* stream) {
+    KernParams kp;
+    int rc = check_sequential(x_train, y_train, n_train, x_query, n_query, d, ny, nn, m, kernel,
+                              &kp);
+    if (rc) return rc;
+    if (output != NNGP_COND_MEAN && output != NNGP_COND_STD && output != NNGP_COND_COV)
+        return NNGP_ERR_INVALID_ARGUMENT;
+    if (n_query > 0 && (!out_mean || (output == NNGP_COND_STD && !out_std) ||
+                        (output == NNGP_COND_COV && !out_cov)))
+        return NNGP_ERR_INVALID_ARGUMENT;
+    CondProblem P{};
+    P.x_train = x_train;
+    P.y_train = y_train;
+    P.n_train = n_train;
+    P.x_query = x_query;
+    P.n_query = n_query;
+    P.d = d;
+    P.ny = ny;
+    P.m = m;
+    P.nn = nn;
+    P.kp = kp;
+    P.alpha = alpha;
+    P.output = output == NNGP_COND_MEAN ? COND_MEAN : (output == NNGP_COND_STD ? COND_STD : COND_COV);
+    P.out_mean = out_mean;
+    P.out_std = out_std;
+    P.out_cov = out_cov;
+    return cond_run(P, workspace, workspace_bytes, (hipStream_t)stream);
+}
+
+size_t nngp_sample_sequential_workspace_bytes(int64_t n_train, int64_t n_query, int m, int d,
+                                              int ny) {
+    if (n_train < 0 || n_query < 0 || m < 1 || d < 1 || ny < 1) return 0;
+    return cond_workspace_bytes(n_train, n_query, m, d, ny, COND_SAMPLE);
+}
+
+int nngp_sample_sequential(const double* x_train, const double* y_train, int64_t n_train,
+                           const double* x_query, int64_t n_query, int d, int ny,
+                           const int32_t* nn, int m, const nngp_kernel* kernel, double alpha,
+                           int n_samples, const double* normals, double* out_samples,
+                           void* workspace, size_t workspace_bytes, void* stream) {
+    KernParams kp;
+    int rc = check_sequential(x_train, y_train, n_train, x_query, n_query, d, ny, nn, m, kernel,
+                              &kp);
+    if (rc) return rc;
+    if (n_samples < 1 || (n_query > 0 && (!normals || !out_samples)))
+        return NNGP_ERR_INVALID_ARGUMENT;
+    CondProblem P{};
+    P.x_train = x_train;
+    P.y_train = y_train;
+    P.n_train = n_train;
+    P.x_query = x_query;
+    P.n_query = n_query;
+    P.d = d;
+    P.ny = ny;
+    P.m = m;
+    P.nn = nn;
+    P.kp = kp;
+    P.alpha = alpha;
+    P.output = COND_SAMPLE;
+    P.n_samples = n_samples;
+    P.normals = normals;
+    P.out_samples = out_samples;
+    return cond_run(P, workspace, workspace_bytes, (hipStream_t)stream);
 }
 
 int nngp_model_create(const double* x_train, const double* y_train, int64_t n_train, int d, int ny,

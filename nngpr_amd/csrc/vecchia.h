@@ -69,6 +69,11 @@ struct VecArgs {
     double* mean;          // npts x ny (predict)
     double* stdv;          // npts (predict)
     WsHeader* hdr;
+    // Generic kernel only (conditioning weights of a sequential prediction):
+    int alpha_cut;         // 1: alpha only on neighbours with index < alpha_limit
+    int64_t alpha_limit;
+    double* wout;          // npts x m: K^-1 k, or NULL
+    double* s2out;         // npts: K_00 - k.K^-1 k (unclamped), or NULL
 };
 
 size_t generic_ws_doubles_per_point(int m);

@@ -72,7 +72,7 @@ __global__ __launch_bounds__(256) void vecchia_generic_kernel(VecArgs a, int64_t
                 v = (i == j) ? 1.0 : 0.0;
             } else {
                 v = cov_entry(kp, a.x + (int64_t)ji * d, a.x + (int64_t)jj * d, d, &Ed, &ud);
-                if (i == j) v += kp.w + a.alpha;
+                if (i == j) v += kp.w + ((a.alpha_cut && ji >= a.alpha_limit) ? 0.0 : a.alpha);
             }
             LP(i, j) = v;
         }
@@ -117,14 +117,17 @@ __global__ __launch_bounds__(256) void vecchia_generic_kernel(VecArgs a, int64_t
     const double s2 = c - zz;
     const double sig = sqrt(s2);
     if (a.stdv) a.stdv[p] = sig;
-    // b = L^-T z_k (gradient only)
-    if (a.grad) {
+    if (a.s2out) a.s2out[p] = s2;
+    // b = L^-T z_k (gradient and conditioning weights)
+    if (a.grad || a.wout) {
         for (int i = m - 1; i >= 0; --i) {
             double sum = KV(i);
             for (int k = i + 1; k < m; ++k) sum -= LP(k, i) * BV(k);
             BV(i) = sum / LP(i, i);
         }
         for (int i = 0; i < m; ++i) AT(i) = 0.0;
+        if (a.wout)
+            for (int i = 0; i < m; ++i) a.wout[p * m + i] = BV(i);
     }
     double ll = 0.0, Gs = 0.0;
     const double half_log_2pi = 0.91893853320467274178;

@@ -210,6 +210,84 @@ def predict(x_train, y_train, x_query, nn, kstruct, alpha, precision=PRECISION_F
     return mean, std
 
 
+COND_MEAN = nat.COND_MEAN
+COND_STD = nat.COND_STD
+COND_COV = nat.COND_COV
+
+
+def _sequential_inputs(x_train, y_train, x_query, nn):
+    x_query = _f64(x_query)
+    nn = _i32(nn)
+    dev = _require_device(x_train, y_train, x_query, nn)
+    nq, d = x_query.shape
+    if x_train is None or x_train.shape[0] == 0:
+        ny = 1 if y_train is None or y_train.ndim == 1 else max(1, y_train.shape[1])
+        x_train = torch.empty((0, d), dtype=torch.float64, device=dev)
+        y_train = torch.empty((0, ny), dtype=torch.float64, device=dev)
+    else:
+        x_train = _f64(x_train)
+        y_train = _f64(y_train)
+        if y_train.ndim == 1:
+            y_train = y_train[:, None]
+    if nn.ndim != 2 or nn.shape[0] != nq:
+        raise ValueError("nn must have one row per query")
+    if nn.shape[1] == 0:  # no neighbours at all: one empty slot per row
+        nn = torch.full((nq, 1), -1, dtype=torch.int32, device=dev)
+    return x_train, y_train, x_query, nn, dev
+
+
+def predict_sequential(x_train, y_train, x_query, nn, kstruct, alpha, output=COND_STD,
+                       ws_slot="sequential"):
+    """Prediction with sequential conditioning (nngp_predict_sequential).
+
+    nn (nq, m) indexes [x_train; x_query] (training rows, then earlier queries).
+    Returns mean (nq, ny), std (nq,) or None, cov (nq, nq) or None.
+    """
+    x_train, y_train, x_query, nn, dev = _sequential_inputs(x_train, y_train, x_query, nn)
+    nt, d = x_train.shape
+    ny = y_train.shape[1]
+    nq = x_query.shape[0]
+    m = nn.shape[1]
+    mean = torch.empty((nq, ny), dtype=torch.float64, device=dev)
+    std = torch.empty(nq, dtype=torch.float64, device=dev) if output != COND_MEAN else None
+    cov = torch.empty((nq, nq), dtype=torch.float64, device=dev) if output == COND_COV else None
+    L = nat.lib()
+    nbytes = L.nngp_predict_sequential_workspace_bytes(nt, nq, m, d, ny, output)
+    ws = workspace(dev, nbytes, ws_slot)
+    rc = L.nngp_predict_sequential(_p(x_train) if nt else None, _p(y_train) if nt else None, nt,
+                                   _p(x_query), nq, d, ny, _p(nn), m, ctypes.byref(kstruct),
+                                   float(alpha), int(output), _p(mean), _p(std), _p(cov), _p(ws),
+                                   ws.numel(), _stream(dev))
+    nat.check(rc, "nngp_predict_sequential")
+    return mean, std, cov
+
+
+def sample_sequential(x_train, y_train, x_query, nn, kstruct, alpha, normals,
+                      ws_slot="sequential"):
+    """Samples of the sequential process (nngp_sample_sequential); normals
+    (nq, ny, n_samples) are standard normal draws.  Returns (nq, ny, n_samples)."""
+    x_train, y_train, x_query, nn, dev = _sequential_inputs(x_train, y_train, x_query, nn)
+    normals = _f64(normals)
+    _require_device(normals, x_query)
+    nt, d = x_train.shape
+    ny = y_train.shape[1]
+    nq = x_query.shape[0]
+    m = nn.shape[1]
+    if normals.ndim != 3 or normals.shape[0] != nq or normals.shape[1] != ny:
+        raise ValueError("normals must have shape (n_query, n_targets, n_samples)")
+    ns = normals.shape[2]
+    out = torch.empty((nq, ny, ns), dtype=torch.float64, device=dev)
+    L = nat.lib()
+    nbytes = L.nngp_sample_sequential_workspace_bytes(nt, nq, m, d, ny)
+    ws = workspace(dev, nbytes, ws_slot)
+    rc = L.nngp_sample_sequential(_p(x_train) if nt else None, _p(y_train) if nt else None, nt,
+                                  _p(x_query), nq, d, ny, _p(nn), m, ctypes.byref(kstruct),
+                                  float(alpha), ns, _p(normals), _p(out), _p(ws), ws.numel(),
+                                  _stream(dev))
+    nat.check(rc, "nngp_sample_sequential")
+    return out
+
+
 class knn_method:
     """Context manager selecting the process-wide kNN search method:
     `with hip_ops.knn_method(hip_ops.KNN_BRUTE_FORCE): ...`."""

@@ -108,10 +108,10 @@ def test_small_m_fit_predict_runs_and_improves():
 def test_unsupported_paths_raise():
     x, x_eval, y = data()
     gpr = make(TorchNngpr, kernel=kern(True), num_nn=10).fit(x, y[:, 0])
-    with pytest.raises(NotImplementedError):
-        gpr.predict(x_eval, return_cov=True)
-    with pytest.raises(NotImplementedError):
-        gpr.predict(x_eval, conditioning_method="full")
+    with pytest.raises(RuntimeError):
+        gpr.predict(x_eval, return_std=True, return_cov=True)
+    with pytest.raises(ValueError):
+        gpr.predict(x_eval, conditioning_method="nearest")
     m = TorchNngpr("cuda", kernel=bk.RBF(1.0) + bk.RBF(2.0), num_nn=5, optimizer=None)
     with pytest.raises(NotImplementedError):
         m.fit(x, y[:, 0])
