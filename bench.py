@@ -188,6 +188,64 @@ def cpu_baseline(ds):
     }
 
 
+def cpu_sequential(ds, n_sample=2000):
+    """The reference's sequential 'full' conditioning loop (restated, one query at a
+    time) on the first n_sample test queries, extrapolated per query."""
+    from oracle import sequential_oracle as so
+    x, y, xq = ds["x_train"], ds["y_train"], ds["x_test"][:n_sample]
+    nn = so.conditioning_nn(x, xq, 0.1, ds["num_nn"], "full")
+    t0 = time.perf_counter()
+    so.predict_sequential(x, y, xq, nn, 0.01, 0.0, 1.0, 0.1, ds["alpha"], "std")
+    per_q = (time.perf_counter() - t0) / n_sample
+    return {"queries_per_s": 1.0 / per_q, "cores": 1, "kind": "port",
+            "sample": "first %d test queries, conditioning_method='full', mean + std "
+                      "(kNN not timed)" % n_sample}
+
+
+def sequential_extras(x, y, xq, ks, ls, m, alpha, steps):
+    """Sequential conditioning on the test set: 'full' mean + std over all test
+    queries (kNN over [train; test] with the cut + nngp_predict_sequential), the
+    joint covariance of the first 4,096 queries, and 16 samples per query."""
+    import numpy as np
+    import torch
+    from nngpr_amd import hip_ops
+    nt = x.shape[0]
+    F32 = hip_ops.PRECISION_FP32_REFINED
+
+    def nn_full(q):
+        return hip_ops.knn(torch.cat([x, q]), q, ls, m, ref_cut=nt, precision=F32)
+
+    def run(q, output):
+        return hip_ops.predict_sequential(x, y, q, nn_full(q), ks, alpha, output)
+
+    def timed_call(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    nq = xq.shape[0]
+    nn = nn_full(xq).cpu().numpy()
+    level = np.zeros(nq, dtype=np.int64)
+    for i in range(nq):
+        q = nn[i][nn[i] >= nt] - nt
+        level[i] = level[q].max() + 1 if q.size else 0
+    t_std = timed_call(lambda: run(xq, hip_ops.COND_STD))
+    t_mean = timed_call(lambda: run(xq, hip_ops.COND_MEAN))
+    xc = xq[:4096]
+    t_cov = timed_call(lambda: run(xc, hip_ops.COND_COV))
+    z = torch.randn((nq, 1, 16), dtype=torch.float64, device=x.device)
+    t_smp = timed_call(lambda: hip_ops.sample_sequential(x, y, xq, nn_full(xq), ks, alpha, z))
+    return {"queries": int(nq), "dependency_levels": int(level.max() + 1),
+            "full_mean_std_ms": t_std * 1e3, "full_mean_std_queries_per_s": nq / t_std,
+            "full_mean_ms": t_mean * 1e3,
+            "full_cov_4096_ms": t_cov * 1e3,
+            "full_sample_16_ms": t_smp * 1e3}
+
+
 def serve_sweep(ds, rates, seconds=3.0, max_requests=300):
     """Poisson request streams against the resident model: batch sizes
     log-uniform in [1, 65536], queries drawn from the test points; returns p50/p99
@@ -247,8 +305,11 @@ def main():
     from nngpr_amd import datasets
     ds = datasets.road_mixture(n_train=args.n_train, n_test=args.n_test)
     cpu = None
+    cpu_seq = None
     if not args.no_cpu:
         cpu = cpu_baseline(ds)  # before the GPU is touched (fork-based pool)
+        if not args.skip_extras:
+            cpu_seq = cpu_sequential(ds)
 
     import torch
     from nngpr_amd import _native as nat
@@ -408,6 +469,11 @@ def main():
         serving = serve_sweep(ds, (50, 100, 200, 500, 1000, 2000))
         extras["correlated_gauss_d18"] = c3_fit()
         extras["uniform_8d_2m"] = [c4_eval(32), c4_eval(64)]
+        seq = sequential_extras(x, y, xq, ks, ls, m, alpha, max(2, args.steps // 2))
+        if cpu_seq is not None:
+            seq["cpu_reference_loop"] = cpu_seq
+            seq["speedup_vs_cpu_loop"] = seq["full_mean_std_queries_per_s"] / cpu_seq["queries_per_s"]
+        extras["sequential_conditioning"] = seq
 
     line = {
         "metric": "loglik+grad evals/s (reference default precision 'only-nn': FP64 factor, "
