@@ -25,7 +25,8 @@ extern "C" {
 
 #define NNGP_ABI_VERSION 2
 #define NNGP_MAX_DIM 64   /* maximum number of input features */
-#define NNGP_MAX_NN 256   /* maximum neighbours per point */
+#define NNGP_MAX_NN 1024  /* maximum neighbours per point (fill, fused and sequential entry points) */
+#define NNGP_MAX_K 256    /* maximum k of a kNN search (nngp_knn, nngp_model_create) */
 
 typedef enum {
     NNGP_OK = 0,
@@ -98,7 +99,7 @@ int nngp_abi_version(void);
  *   ref (n_ref x d), query (n_query x d): FP64 row-major device arrays.
  *   scale (host, n_scale doubles): coordinates are divided by it before the
  *     search (kernel-space search with RBF length scales); n_scale 0 = raw.
- *   k neighbours per query; has_cut != 0 limits query i to refs [0, i + ref_cut)
+ *   k (1..NNGP_MAX_K) neighbours per query; has_cut != 0 limits query i to refs [0, i + ref_cut)
  *     (the reference's ref_cut_index; None <=> has_cut == 0).
  *   out_idx (n_query x k int32, device): ascending (key, index) order, -1 padded.
  * Key = FP64 squared distance of scaled coordinates, accumulated with FMA over

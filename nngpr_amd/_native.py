@@ -12,7 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libnngp_hip.so")
 
 NNGP_MAX_DIM = 64
-NNGP_MAX_NN = 256
+NNGP_MAX_NN = 1024
+NNGP_MAX_K = 256
 
 NNGP_OK = 0
 NNGP_ERR_INVALID_ARGUMENT = -1

@@ -33,6 +33,26 @@ from .array_module import TorchModule
 from .kernel_params import KernelDecomposition
 
 
+def _knn_lists(ref, query, scale, k, ref_cut=None, precision=hip_ops.PRECISION_FP32_REFINED):
+    """k nearest refs of every query (hip_ops.knn).  Above the search's k limit the
+    lists are accepted only when they hold every candidate ref of every query (the
+    exact-GP settings, num_nn >= the number of rows): then each query lists all of
+    its candidate refs in index order, the same set as the search would return
+    (refs with a non-finite coordinate included)."""
+    if k <= nat.NNGP_MAX_K:
+        return hip_ops.knn(ref, query, scale, k, ref_cut=ref_cut, precision=precision)
+    n_ref, nq = ref.shape[0], query.shape[0]
+    dev = query.device
+    avail = torch.full((nq,), n_ref, dtype=torch.int64, device=dev)
+    if ref_cut is not None:
+        avail = torch.clamp(torch.arange(nq, device=dev) + int(ref_cut), 0, n_ref)
+    if nq and k < int(avail.max().item()):
+        raise NotImplementedError("num_nn above %d is supported only when it covers every "
+                                  "candidate row (%d here)" % (nat.NNGP_MAX_K, int(avail.max())))
+    idx = torch.arange(k, device=dev)[None, :]
+    return torch.where(idx < avail[:, None], idx, -1).to(torch.int32).contiguous()
+
+
 def _precisions(allow_downcast_f32):
     """(kNN precision, fill/factor precision) for an allow_downcast_f32 setting.
 
@@ -107,7 +127,7 @@ class HipWorker:
         """base_nngpr.py:1613-1686 semantics on the HIP kNN (exact FP64 keys)."""
         ref = self.xp.from_numpy(ref)
         query = self.xp.from_numpy(query)
-        return hip_ops.knn(ref, query, self.knn_scale(kernel, nn_type), int(num_nn),
+        return _knn_lists(ref, query, self.knn_scale(kernel, nn_type), int(num_nn),
                            ref_cut=ref_cut_index, precision=precision)
 
     def fill_nn_kernel(self, x, nn_indices, kernel, eval_gradient=False):
@@ -204,6 +224,12 @@ class Nngpr(GaussianProcessRegressor):
         return batched_kernels.ConstantKernel(1.0, constant_value_bounds="fixed") * \
             batched_kernels.RBF(1.0, length_scale_bounds="fixed")
 
+    def __getstate__(self):
+        """Pickled without the device-resident copies (rebuilt on first use)."""
+        state = super().__getstate__()
+        state.pop("_dev_cache", None)
+        return state
+
     def _device_train(self):
         """Training arrays resident on the device (uploaded once per fit)."""
         key = (id(self.X_train_), id(self.y_train_))
@@ -228,7 +254,7 @@ class Nngpr(GaussianProcessRegressor):
         nn = knn_cache.get(key)
         if nn is None:
             knn_cache.clear()
-            nn = hip_ops.knn(x_dev, x_dev, scale, k, ref_cut=0, precision=precision)
+            nn = _knn_lists(x_dev, x_dev, scale, k, ref_cut=0, precision=precision)
             knn_cache[key] = nn
         return nn
 
@@ -356,7 +382,7 @@ class Nngpr(GaussianProcessRegressor):
         xq = xp.from_numpy(np.ascontiguousarray(X, dtype=np.float64))
         nt = x_dev.shape[0]
         k = max(1, min(nt, self.num_nn))
-        nn = hip_ops.knn(x_dev, xq, worker.knn_scale(kernel, self.nn_type), k, ref_cut=None,
+        nn = _knn_lists(x_dev, xq, worker.knn_scale(kernel, self.nn_type), k, ref_cut=None,
                          precision=knn_prec)
         mean_d, std_d = hip_ops.predict(x_dev, y_dev, xq, nn, ks, float(self.alpha),
                                         precision=fac_prec, return_std=return_std)
@@ -387,18 +413,18 @@ class Nngpr(GaussianProcessRegressor):
         empty = torch.empty((nq, 0), dtype=torch.int32, device=dev)
         if method == "only-train":
             k = min(nt, self.num_nn)
-            return hip_ops.knn(x_train, xq, scale, k, precision=knn_prec) if k > 0 else empty
+            return _knn_lists(x_train, xq, scale, k, precision=knn_prec) if k > 0 else empty
         if method == "full":
             k = min(self.num_nn, nt + nq)
             x_full = torch.cat([x_train, xq], dim=0)
-            return hip_ops.knn(x_full, xq, scale, k, ref_cut=nt, precision=knn_prec)
+            return _knn_lists(x_full, xq, scale, k, ref_cut=nt, precision=knn_prec)
         kq = min(self.num_nn, nq - 1)
-        qn = hip_ops.knn(xq, xq, scale, kq, ref_cut=0, precision=knn_prec) if kq > 0 else empty
+        qn = _knn_lists(xq, xq, scale, kq, ref_cut=0, precision=knn_prec) if kq > 0 else empty
         qn = torch.where(qn >= 0, qn + nt, qn)
         if nt == 0:
             return qn
         kt = min(self.num_nn, nt)
-        tr = hip_ops.knn(x_train, xq, scale, kt, precision=knn_prec)
+        tr = _knn_lists(x_train, xq, scale, kt, precision=knn_prec)
         # valid training entries, then valid query entries, then -1 (the extra last
         # column takes the scatter of the empty query slots)
         merged = torch.full((nq, kt + kq + 1), -1, dtype=torch.int32, device=dev)

@@ -504,7 +504,7 @@ int nngp_model_create(const double* x_train, const double* y_train, int64_t n_tr
     int rc = to_kern_params(kernel, d, &kp);
     if (rc) return rc;
     const int m = (int)(num_nn < n_train ? num_nn : n_train);
-    if (m > NNGP_MAX_NN) return NNGP_ERR_UNSUPPORTED;
+    if (m > NNGP_MAX_K) return NNGP_ERR_UNSUPPORTED;
     Carver c(nullptr, 0);
     c.take<double>((size_t)n_train * d);
     c.take<double>((size_t)n_train * ny);

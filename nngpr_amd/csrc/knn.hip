@@ -917,7 +917,7 @@ int knn_search(const KnnRefs& R, const double* query, int64_t n_query, int k, in
                int64_t ref_cut, int32_t* out_idx, void* ws, size_t ws_bytes, WsHeader* hdr,
                hipStream_t st) {
     const int d = R.d;
-    if (k < 1 || k > NNGP_MAX_NN || n_query < 0) return NNGP_ERR_INVALID_ARGUMENT;
+    if (k < 1 || k > NNGP_MAX_K || n_query < 0) return NNGP_ERR_INVALID_ARGUMENT;
     const bool same = query == nullptr;
     Carver c(ws, ws_bytes);
     QueryMem Q = carve_query(c, R.n, n_query, d, k, R.precision, same, has_cut != 0);
@@ -1130,7 +1130,7 @@ int knn_run(const double* ref, int64_t n_ref, const double* query, int64_t n_que
             int32_t* out_idx, void* ws, size_t ws_bytes, hipStream_t st, bool reset_stats) {
     int rc = check_common(n_ref, d, n_scale, precision);
     if (rc) return rc;
-    if (k < 1 || k > NNGP_MAX_NN || n_query < 0) return NNGP_ERR_INVALID_ARGUMENT;
+    if (k < 1 || k > NNGP_MAX_K || n_query < 0) return NNGP_ERR_INVALID_ARGUMENT;
     if (!ws) return NNGP_ERR_WORKSPACE_TOO_SMALL;
     const bool same = (ref == query && n_ref == n_query);
     Carver c(ws, ws_bytes);

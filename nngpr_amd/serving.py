@@ -83,6 +83,16 @@ class ResidentModel:
                    num_nn=est.num_nn, nn_type=est.nn_type, precision=precision,
                    y_mean=est._y_train_mean, y_std=est._y_train_std, device=dev)
 
+    @classmethod
+    def from_pickle(cls, path, precision=None, device=None):
+        """Resident copy of a fitted estimator saved with pickle (the estimator's own
+        format, as the reference's estimators are saved: torch_nngpr.py:214-215).
+        Unpickling runs code named in the file: load trusted files only."""
+        import pickle
+        with open(path, "rb") as f:
+            est = pickle.load(f)
+        return cls.from_estimator(est, precision=precision, device=device)
+
     def workspace_bytes(self, max_query):
         return int(nat.lib().nngp_model_predict_workspace_bytes(self._handle, int(max_query)))
 

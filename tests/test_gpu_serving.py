@@ -126,3 +126,29 @@ def test_from_estimator_matches_predict():
     np.testing.assert_allclose(mean[:, 0], mean_e, rtol=1e-12, atol=1e-12)
     np.testing.assert_allclose(std[:, 0], std_e, rtol=1e-12)
     model.close()
+
+
+def test_pickled_estimator_round_trip_and_serving(tmp_path):
+    """A fitted estimator survives pickling (without its device copies) and loads
+    into the resident serving model."""
+    import pickle
+    ds = datasets.small_uniform()
+    k = bk.WhiteKernel(0.01) + bk.ConstantKernel(1.0) * bk.RBF(0.3)
+    est = TorchNngpr("cuda:0", kernel=k, num_nn=16, optimizer=None, normalize_y=True,
+                     allow_downcast_f32="no").fit(ds["x_train"], ds["y_train"])
+    mean_e, std_e = est.predict(ds["x_test"], return_std=True)
+    blob = pickle.dumps(est)
+    assert b"_dev_cache" not in blob
+    est2 = pickle.loads(blob)
+    mean2, std2 = est2.predict(ds["x_test"], return_std=True)
+    np.testing.assert_array_equal(mean2, mean_e)
+    np.testing.assert_array_equal(std2, std_e)
+    path = tmp_path / "model.pkl"
+    path.write_bytes(blob)
+    model = ResidentModel.from_pickle(str(path))
+    try:
+        mean, std = model.predict(ds["x_test"])
+    finally:
+        model.close()
+    np.testing.assert_allclose(mean[:, 0], mean_e, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(std[:, 0], std_e, rtol=1e-12)
