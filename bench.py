@@ -40,6 +40,9 @@ def parse():
     ap.add_argument("--n-test", type=int, default=43_487)
     ap.add_argument("--skip-extras", action="store_true",
                     help="only the headline timed region (used for profiling runs)")
+    ap.add_argument("--extras", default=None,
+                    help="comma-separated subset of the extras to run: modes, serving, "
+                         "fit_d18, uniform_8d, matern, sequential (default: all)")
     ap.add_argument("--copy-probe-bytes", type=int, default=0,
                     help="run only a plain device-to-device copy of this many bytes, 5 times "
                          "(calibrates the HBM byte counters against a known traffic)")
@@ -306,9 +309,14 @@ def main():
     ds = datasets.road_mixture(n_train=args.n_train, n_test=args.n_test)
     cpu = None
     cpu_seq = None
+    ALL_EXTRAS = {"modes", "serving", "fit_d18", "uniform_8d", "matern", "sequential"}
+    want = set() if args.skip_extras else (
+        ALL_EXTRAS if args.extras is None else set(args.extras.split(",")))
+    if want - ALL_EXTRAS:
+        raise SystemExit("unknown extras: %s" % ", ".join(sorted(want - ALL_EXTRAS)))
     if not args.no_cpu:
         cpu = cpu_baseline(ds)  # before the GPU is touched (fork-based pool)
-        if not args.skip_extras:
+        if "sequential" in want:
             cpu_seq = cpu_sequential(ds)
 
     import torch
@@ -410,7 +418,7 @@ def main():
                 "algorithmic_flop_per_point": fused_flop_pt}
 
     extras = {"stage_ms": stage_ms(stages)}
-    if not args.skip_extras:
+    if "modes" in want:
         nq = xq.shape[0]
         res = {}
         for mode in MODES:
@@ -465,10 +473,30 @@ def main():
         })
 
     serving = None
-    if not args.skip_extras:
+    if "serving" in want:
         serving = serve_sweep(ds, (50, 100, 200, 500, 1000, 2000))
+    if "fit_d18" in want:
         extras["correlated_gauss_d18"] = c3_fit()
+    if "uniform_8d" in want:
         extras["uniform_8d_2m"] = [c4_eval(32), c4_eval(64)]
+    if "matern" in want:
+        # the same evaluation with a Matern stationary part (same parameters)
+        matern = {}
+        for nu in (0.5, 1.5, 2.5):
+            km = nat.kernel_struct(dict(params, matern_nu=nu), mapping, len(kern.theta))
+
+            def ev_m(km=km):
+                nn_m = hip_ops.knn(x, x, ls, m, ref_cut=0, precision=F32)
+                return hip_ops.vecchia_loglik(x, y, nn_m, km, alpha, precision=F64,
+                                              eval_gradient=True, check_pd=False)
+            ev_m()
+            t_m, st_m = stage_times(ev_m, max(2, args.steps // 2))
+            matern["nu=%g" % nu] = {"evals_per_s": 1.0 / t_m,
+                                    "fused_ms": st_m["fused_loglik_grad"] /
+                                    max(st_m["fused_loglik_grad_launches"], 1),
+                                    "loglik": float(ev_m()[0].item())}
+        extras["matern"] = matern
+    if "sequential" in want:
         seq = sequential_extras(x, y, xq, ks, ls, m, alpha, max(2, args.steps // 2))
         if cpu_seq is not None:
             seq["cpu_reference_loop"] = cpu_seq
