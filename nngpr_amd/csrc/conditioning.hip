@@ -32,6 +32,7 @@
 #include "common.h"
 #include "conditioning.h"
 #include "vecchia.h"
+#include "vecchia_reg.h"
 
 namespace nngp {
 
@@ -365,7 +366,9 @@ int cond_run(const CondProblem& P, void* ws, size_t ws_bytes, hipStream_t st) {
     a.alpha_limit = nt;
     a.wout = L.w;
     a.s2out = L.s2;
-    launch_generic(a, L.wsL, L.wsv, L.P, st);
+    // the segment kernel where it applies (m <= 64, d <= 32, LDS within the block
+    // limit), else the generic kernel
+    if (launch_seg_weights(a, st) != 0) launch_generic(a, L.wsL, L.wsv, L.P, st);
     NNGP_HIP_CHECK(hipGetLastError());
 
     // ---- level by level

@@ -24,5 +24,8 @@ int launch_seg(const VecArgs& a, const RegExtra& ex, bool fp32, int64_t max_poin
                hipStream_t st);
 int launch_seg_matern(const VecArgs& a, const RegExtra& ex, bool fp32, int64_t max_points,
                       hipStream_t st);
+// FP64 conditioning weights of a sequential prediction (a.wout, a.s2out, alpha
+// only on neighbours below a.alpha_limit) for every point of a.
+int launch_seg_weights(const VecArgs& a, hipStream_t st);
 
 }  // namespace nngp

@@ -235,7 +235,10 @@ __device__ __forceinline__ SegEntry seg_entry(int e, int i0, int i1) {
 
 // One group of G points starting at slot `base`.  Coordinates are held with DM
 // features (DM >= d, zero padded).  MAT: Matern stationary part (kp.fam), else RBF.
-template <typename T, int MR, int DM, bool MAT>
+// CW (FP64 prediction only): conditioning weights of a sequential prediction -
+// alpha only on neighbours with index < a.alpha_limit, and w = K^-1 k and
+// s2 = K_00 - k.w written to a.wout / a.s2out next to the mean.
+template <typename T, int MR, int DM, bool MAT, bool CW = false>
 __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, int64_t base,
                                          int64_t count) {
     constexpr int S = MR / 2;
@@ -386,7 +389,12 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
     // are never used before being cleared below.  Diagonal: K_ii + w + alpha for
     // a valid row, 1 for a padding row.
     const double kd = fma(kp.s, 1.0, kp.C) + diag_add;
-    const double kd0 = v0 ? kd : 1.0, kd1 = v1 ? kd : 1.0;
+    double kd0 = v0 ? kd : 1.0, kd1 = v1 ? kd : 1.0;
+    if constexpr (CW) {
+        const double kw = fma(kp.s, 1.0, kp.C) + kp.w;
+        kd0 = v0 ? kw + (j0 < a.alpha_limit ? a.alpha : 0.0) : 1.0;
+        kd1 = v1 ? kw + (j1 < a.alpha_limit ? a.alpha : 0.0) : 1.0;
+    }
     T L0[S], L1[MR];
     {
         const double* r0p = Kt + stri(i0, 0);
@@ -528,6 +536,13 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
         s2 = c - seg_sum<S>(zk0 * zk0 + zk1 * zk1);
         sig = sqrt(s2);
         const double mu0 = seg_sum<S>(zk0 * (double)r0[1] + zk1 * (double)r1[1]);
+        if constexpr (CW) {
+            T u0[1] = {r0[0]}, u1[1] = {r1[0]};
+            seg_bwd<T, MR, 1>(L0, L1, dv0, dv1, u0, u1, t);
+            if (live && i0 < a.m) a.wout[p * a.m + i0] = (double)u0[0];
+            if (live && i1 < a.m) a.wout[p * a.m + i1] = (double)u1[0];
+            if (live && t == 0) a.s2out[p] = s2;
+        }
         double av0 = 0.0, av1 = 0.0;
         if (with_grad) {
             seg_bwd<T, MR, 2>(L0, L1, dv0, dv1, r0, r1, t);
@@ -686,7 +701,8 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
 // kept live across groups).
 // (MR = 64: the LDS of a block keeps occupancy under one wave per SIMD, so the
 // register budget is a whole SIMD's.)
-// (RBF and Matern kernels carry distinct names so that profiles tell them apart.)
+// (RBF, Matern and conditioning-weight kernels carry distinct names so that
+// profiles tell them apart.)
 template <typename T, int MR, int DM, bool MAT>
 __device__ __forceinline__ void seg_group(const VecArgs& a, const RegExtra& ex) {
     constexpr int G = 64 / (MR / 2);
@@ -723,6 +739,14 @@ __global__ __launch_bounds__(64) void vecchia_seg_list_kernel(VecArgs a, RegExtr
 template <typename T, int MR, int DM>
 __global__ __launch_bounds__(64) void vecchia_seg_matern_list_kernel(VecArgs a, RegExtra ex) {
     seg_list<T, MR, DM, true>(a, ex);
+}
+// Conditioning weights (FP64 prediction pass, RBF or Matern).
+template <int MR, int DM, bool MAT>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MR <= 32 ? 2 : 1)))
+void vecchia_seg_weights_kernel(VecArgs a) {
+    constexpr int G = 64 / (MR / 2);
+    const int64_t base = (int64_t)blockIdx.x * G;
+    if (base < a.npts) seg_body<double, MR, DM, MAT, true>(a, RegExtra{}, base, a.npts);
 }
 
 template <typename T, int MR, int DM>
@@ -764,6 +788,28 @@ void launch_seg_d(const VecArgs& a, const RegExtra& ex, int64_t max_points, hipS
     if (a.d <= 3) launch_seg_mr<T, MR, 3, MAT>(a, ex, max_points, st);
     else if (a.d <= 8) launch_seg_mr<T, MR, 8, MAT>(a, ex, max_points, st);
     else launch_seg_mr<T, MR, kSegMaxDim, MAT>(a, ex, max_points, st);
+}
+
+template <int MR, bool MAT>
+void launch_seg_weights_mr(const VecArgs& a, hipStream_t st) {
+    constexpr int G = 64 / (MR / 2);
+    const unsigned blocks = (unsigned)((a.npts + G - 1) / G);
+    if (a.d <= 3)
+        hipLaunchKernelGGL((vecchia_seg_weights_kernel<MR, 3, MAT>), dim3(blocks), dim3(64),
+                           (seg_lds<double, MR, 3>()), st, a);
+    else if (a.d <= 8)
+        hipLaunchKernelGGL((vecchia_seg_weights_kernel<MR, 8, MAT>), dim3(blocks), dim3(64),
+                           (seg_lds<double, MR, 8>()), st, a);
+    else
+        hipLaunchKernelGGL((vecchia_seg_weights_kernel<MR, kSegMaxDim, MAT>), dim3(blocks),
+                           dim3(64), (seg_lds<double, MR, kSegMaxDim>()), st, a);
+}
+
+template <bool MAT>
+void launch_seg_weights_family(const VecArgs& a, hipStream_t st) {
+    if (a.m <= 16) launch_seg_weights_mr<16, MAT>(a, st);
+    else if (a.m <= 32) launch_seg_weights_mr<32, MAT>(a, st);
+    else launch_seg_weights_mr<64, MAT>(a, st);
 }
 
 // fp32: FP32 factor + FP64 refinement form; else the FP64 form.
