@@ -65,6 +65,19 @@ def test_same_fit_results(y_dim):
     assert np.max(np.abs(ref.kernel_.theta / gpr.kernel_.theta - 1)) < 1e-6
 
 
+def test_same_fit_results_with_restarts():
+    """n_restarts_optimizer: the same restart draws as scikit-learn (theta0 uniform
+    in the log bounds from random_state), so the best optimum matches."""
+    x, _, y = data()
+    ref = skgp.GaussianProcessRegressor(kernel=kern().bind_to_array_module(NumpyModule()),
+                                        normalize_y=True, n_restarts_optimizer=2,
+                                        random_state=3).fit(x, y[:, 0])
+    gpr = make(TorchNngpr, kernel=kern(), normalize_y=True, num_nn=x.shape[0],
+               n_restarts_optimizer=2, random_state=3).fit(x, y[:, 0])
+    assert abs(gpr.log_marginal_likelihood_value_ / ref.log_marginal_likelihood_value_ - 1) < 1e-8
+    np.testing.assert_allclose(gpr.kernel_.theta, ref.kernel_.theta, rtol=1e-5, atol=1e-6)
+
+
 @pytest.mark.parametrize("cls", [TorchNngpr, NumpyNngpr])
 @pytest.mark.parametrize("use_eval", [True, False])
 @pytest.mark.parametrize("y_dim", [1, 3])
