@@ -159,20 +159,6 @@ __global__ void p2_kernel(const double* u64, int64_t n, int d, const long long* 
     p2[i] = pp;
 }
 
-// Blocked FP64 layout (nv = dp) of n scaled points given as AoS (n x d); entries
-// up to npad are padding (+inf in feature 0, never admitted).
-__global__ void blocked_from_aos_kernel(const double* u64, int64_t n, int64_t npad, int d, int dp,
-                                        double* s64) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= npad) return;
-    for (int f = 0; f < dp; ++f) {
-        double v = 0.0;
-        if (i >= n) v = f == 0 ? __builtin_inf() : 0.0;
-        else if (f < d) v = u64[i * d + f];
-        s64[blk8(i, f, dp)] = v;
-    }
-}
-
 // ---------------------------------------------------------------------------
 // Streaming scan (exact FP64 or FP32 screen)
 // ---------------------------------------------------------------------------
@@ -663,6 +649,65 @@ __global__ void fill_i32_kernel(int32_t* p, int64_t n, int32_t v) {
     if (i < n) p[i] = v;
 }
 
+// Wave-wide bitonic stage over (key, index) pairs, one pair per lane: lanes l and
+// l ^ stride exchange; the lower lane of an ascending block keeps the smaller pair.
+__device__ __forceinline__ void wave_pair_step(double& key, int& idx, int lane, int size,
+                                               int stride) {
+    const double ok = __shfl_xor(key, stride, 64);
+    const int oi = __shfl_xor(idx, stride, 64);
+    const bool up = (lane & size) == 0;
+    const bool lower = (lane & stride) == 0;
+    const bool other_less = pair_less(ok, oi, key, idx);
+    const bool other_more = pair_less(key, idx, ok, oi);
+    if (lower == up ? other_less : other_more) {
+        key = ok;
+        idx = oi;
+    }
+}
+
+// Exact top-k (k <= 64) of query rows [qlo, qlo + gridDim.x) whose ref limit
+// (row + cut, at most a few hundred refs) is below the first grid band: one wave
+// per row, the refs in chunks of 64 lanes (exact_key), each chunk sorted by (key,
+// index) with a bitonic network and merged into the row's 64 smallest so far.
+// Refs or queries with a non-finite coordinate give no key (never neighbours);
+// missing neighbours are -1.
+__global__ __launch_bounds__(64) void prefix_knn_kernel(const double* ru64, const double* qu64,
+                                                        int d, int64_t qlo, int64_t cut,
+                                                        int64_t n_ref, int k, int32_t* out_idx) {
+    const int64_t i = qlo + blockIdx.x;
+    const int lane = threadIdx.x;
+    const double* uq = qu64 + i * d;
+    int64_t lim = i + cut;
+    lim = lim < 0 ? 0 : (lim > n_ref ? n_ref : lim);
+    double bk = __builtin_inf();
+    int bi = -1;
+    for (int64_t base = 0; base < lim; base += 64) {
+        const int64_t r = base + lane;
+        double key = __builtin_inf();
+        int idx = -1;
+        if (r < lim) {
+            const double kr = exact_key(ru64 + r * d, uq, d);
+            if (kr < __builtin_inf()) {  // NaN and +inf are never admitted
+                key = kr;
+                idx = (int)r;
+            }
+        }
+        for (int size = 2; size <= 64; size <<= 1)
+            for (int stride = size >> 1; stride > 0; stride >>= 1)
+                wave_pair_step(key, idx, lane, size, stride);
+        // the 64 smallest of the running list and the chunk: elementwise minimum
+        // against the reversed chunk (a bitonic sequence), then a bitonic merge
+        const double rk = __shfl(key, 63 - lane, 64);
+        const int ri = __shfl(idx, 63 - lane, 64);
+        if (pair_less(rk, ri, bk, bi)) {
+            bk = rk;
+            bi = ri;
+        }
+        for (int stride = 32; stride > 0; stride >>= 1) wave_pair_step(bk, bi, lane, 64, stride);
+    }
+    if (lane < k) out_idx[i * k + lane] = bi;
+}
+
 // ---------------------------------------------------------------------------
 // Host dispatch
 // ---------------------------------------------------------------------------
@@ -792,8 +837,6 @@ struct QueryMem {
     double* lkey;
     int32_t* lidx;
     GridSearchMem grid;
-    double* pre_r;   // grid search with a cut: blocked FP64 refs / queries of the
-    double* pre_q;   //   rows scanned brute force
 };
 
 QueryMem carve_query(Carver& c, int64_t n_ref, int64_t n_query, int d, int k, int precision,
@@ -822,14 +865,7 @@ QueryMem carve_query(Carver& c, int64_t n_ref, int64_t n_query, int d, int k, in
     }
     Q.lkey = c.take<double>((size_t)k * kFallbackThreads);
     Q.lidx = c.take<int32_t>((size_t)k * kFallbackThreads);
-    if (grid_supported(d, k)) {
-        carve_grid_search(c, n_ref, n_query, k, with_cut, &Q.grid);
-        if (with_cut) {
-            const int64_t pre = round_up(grid_prefix_limit(n_ref, k), 64) + 64;
-            Q.pre_r = c.take<double>((size_t)pre * scan_dim(d));
-            Q.pre_q = c.take<double>((size_t)pre * scan_dim(d));
-        }
-    }
+    if (grid_supported(d, k)) carve_grid_search(c, n_ref, n_query, k, with_cut, &Q.grid);
     return Q;
 }
 
@@ -976,29 +1012,10 @@ int knn_search(const KnnRefs& R, const double* query, int64_t n_query, int k, in
                                    0, st, out_idx, tot, -1);
             }
             if (row0 > qlo) {
-                const int64_t nr = std::min<int64_t>(R.n, L0);
-                const int64_t rpad = round_up(nr, 64) + 64;
-                const int64_t qpad = round_up(row0 - qlo, 64) + 64;
-                hipLaunchKernelGGL(blocked_from_aos_kernel, dim3((unsigned)((rpad + 255) / 256)),
-                                   dim3(256), 0, st, R.u64, nr, rpad, d, dp, Q.pre_r);
-                hipLaunchKernelGGL(blocked_from_aos_kernel, dim3((unsigned)((qpad + 255) / 256)),
-                                   dim3(256), 0, st, qu64 + qlo * d, row0 - qlo, qpad, d, dp,
-                                   Q.pre_q);
-                ScanArgs<double> a{};
-                a.ref = Q.pre_r;
-                a.n_ref = (int32_t)nr;
-                a.qry = Q.pre_q;
-                a.n_query = row0 - qlo;
-                a.has_cut = 1;
-                a.cut = ref_cut;
-                a.reverse = 1;
-                a.row_off = qlo;
-                a.k_out = k;
-                a.out_idx = out_idx;
                 timing_begin(TK_KNN_SCAN, st);
-                const int rc = launch_scan<double, false>(a, d, exact_K(k), st);
+                hipLaunchKernelGGL(prefix_knn_kernel, dim3((unsigned)(row0 - qlo)), dim3(64), 0,
+                                   st, R.u64, qu64, d, qlo, ref_cut, R.n, k, out_idx);
                 timing_end(TK_KNN_SCAN, st);
-                if (rc != 0) return NNGP_ERR_UNSUPPORTED;
             }
             NNGP_HIP_CHECK(hipGetLastError());
         }
