@@ -51,18 +51,45 @@ __device__ __forceinline__ bool point_finite(const double* u, int d) {
     return fin;
 }
 
-__global__ void grid_bbox_init_kernel(long long* bb) {
-    const int f = threadIdx.x;
-    if (f < 3) {
-        bb[2 * f] = 0x7fffffffffffffffLL;
-        bb[2 * f + 1] = (long long)0x8000000000000000ULL;
-    }
+constexpr int kBBoxBlocks = 256;  // blocks of the bounding-box pass (one partial each)
+
+__device__ __forceinline__ void bbox_wave_reduce(long long (&lo)[3], long long (&hi)[3]) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            const long long ol = __shfl_xor(lo[f], off, 64);
+            const long long oh = __shfl_xor(hi[f], off, 64);
+            lo[f] = ol < lo[f] ? ol : lo[f];
+            hi[f] = oh > hi[f] ? oh : hi[f];
+        }
 }
 
-// Bounding box of the points whose coordinates are all finite.
+// Reduces the block's 4 waves (after bbox_wave_reduce) into thread 0's lo / hi.
+__device__ __forceinline__ void bbox_block_reduce(long long (&lo)[3], long long (&hi)[3]) {
+    __shared__ long long wl[3][4], wh[3][4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            wl[f][w] = lo[f];
+            wh[f][w] = hi[f];
+        }
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int f = 0; f < 3; ++f)
+            for (int v = 1; v < 4; ++v) {
+                lo[f] = wl[f][v] < lo[f] ? wl[f][v] : lo[f];
+                hi[f] = wh[f][v] > hi[f] ? wh[f][v] : hi[f];
+            }
+}
+
+// Bounding box of the points whose coordinates are all finite, as order-preserving
+// integer keys: one partial (lo, hi per feature) per block, reduced by
+// grid_bbox_final_kernel (no same-address atomics).
 __global__ __launch_bounds__(256) void grid_bbox_kernel(const double* u64, int64_t n, int d,
-                                                        long long* bb) {
-    __shared__ long long slo[3][256], shi[3][256];
+                                                        long long* part) {
     long long lo[3], hi[3];
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
@@ -82,27 +109,40 @@ __global__ __launch_bounds__(256) void grid_bbox_kernel(const double* u64, int64
             }
         }
     }
+    bbox_wave_reduce(lo, hi);
+    bbox_block_reduce(lo, hi);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            part[blockIdx.x * 6 + 2 * f] = lo[f];
+            part[blockIdx.x * 6 + 2 * f + 1] = hi[f];
+        }
+}
+
+// bb = the reduction of nb block partials (empty: lo = max, hi = min).
+__global__ __launch_bounds__(256) void grid_bbox_final_kernel(const long long* part, int nb,
+                                                              long long* bb) {
+    long long lo[3], hi[3];
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
-        slo[f][threadIdx.x] = lo[f];
-        shi[f][threadIdx.x] = hi[f];
+        lo[f] = 0x7fffffffffffffffLL;
+        hi[f] = (long long)0x8000000000000000ULL;
     }
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            for (int f = 0; f < 3; ++f) {
-                slo[f][threadIdx.x] = min(slo[f][threadIdx.x], slo[f][threadIdx.x + s]);
-                shi[f][threadIdx.x] = max(shi[f][threadIdx.x], shi[f][threadIdx.x + s]);
-            }
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            const long long l = part[b * 6 + 2 * f], h = part[b * 6 + 2 * f + 1];
+            lo[f] = l < lo[f] ? l : lo[f];
+            hi[f] = h > hi[f] ? h : hi[f];
         }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        for (int f = 0; f < d; ++f) {
-            atomicMin(&bb[2 * f], slo[f][0]);
-            atomicMax(&bb[2 * f + 1], shi[f][0]);
+    bbox_wave_reduce(lo, hi);
+    bbox_block_reduce(lo, hi);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            bb[2 * f] = lo[f];
+            bb[2 * f + 1] = hi[f];
         }
-    }
 }
 
 // Cell size and grid dimensions from the bounding box, for a grid over n refs.  The cell size targets n / kPointsPerCell cells over the box; it
@@ -998,7 +1038,7 @@ int64_t grid_prefix_limit(int64_t n_ref, int k) {
 }
 
 void carve_grid_refs(Carver& c, int64_t n_ref, GridRefMem* M) {
-    M->bbox = c.take<long long>(8);
+    M->bbox = c.take<long long>(8 + 6 * (size_t)kBBoxBlocks);  // the box, then block partials
     carve_grid(c, n_ref, &M->full);
 }
 
@@ -1027,11 +1067,11 @@ void carve_grid_search(Carver& c, int64_t n_ref, int64_t n_query, int k, bool wi
 int grid_build(const double* u64, int64_t n, int d, const GridRefMem& M, hipStream_t st) {
     if (d < 1 || d > 3) return NNGP_ERR_INVALID_ARGUMENT;
     timing_begin(TK_KNN_GRID_BUILD, st);
-    hipLaunchKernelGGL(grid_bbox_init_kernel, dim3(1), dim3(64), 0, st, M.bbox);
-    if (n > 0) {
-        const unsigned gb = (unsigned)std::min<int64_t>(512, (n + 255) / 256);
-        hipLaunchKernelGGL(grid_bbox_kernel, dim3(gb), dim3(256), 0, st, u64, n, d, M.bbox);
-    }
+    long long* part = M.bbox + 8;
+    const int gb = (int)std::min<int64_t>(kBBoxBlocks, (n + 255) / 256);
+    if (gb > 0)
+        hipLaunchKernelGGL(grid_bbox_kernel, dim3((unsigned)gb), dim3(256), 0, st, u64, n, d, part);
+    hipLaunchKernelGGL(grid_bbox_final_kernel, dim3(1), dim3(256), 0, st, part, gb, M.bbox);
     const SortJob J = ref_job(M.full, n);
     run_sorts(u64, M.bbox, d, &J, 1, true, st);
     timing_end(TK_KNN_GRID_BUILD, st);
