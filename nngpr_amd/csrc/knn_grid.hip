@@ -539,7 +539,9 @@ __device__ __forceinline__ void merge_slots(double (&keys)[K], int (&ids)[K],
 // query to the nearest cell outside the visited block.
 template <int D, int K>
 __global__ __launch_bounds__(64 * kCoopWaves)
-__attribute__((amdgpu_waves_per_eu(K <= 16 ? 3 : 2)))
+// (K = 64: one wave per SIMD, so that the lists live in VGPRs and AGPRs instead of
+// spilling to scratch; 16 % faster on the Vecchia search, 40 % on a query search.)
+__attribute__((amdgpu_waves_per_eu(K <= 16 ? 3 : (K <= 32 ? 2 : 1))))
 void lane_knn_kernel(SearchArgs a) {
     constexpr int B = kCoopSlots;
     constexpr int CH = 4;  // refs per step of a lane
