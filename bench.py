@@ -114,24 +114,27 @@ def c3_fit(steps=50):
 # The reference's published seconds per restart at m=32 on 3droad (n_train subsets,
 # RTX 4070 Super; read from its benchmark plot, +-15 %, benchmark.ipynb:459).
 REFERENCE_SECONDS_PER_RESTART = {
-    200_000: {"TorchNngpr (RTX 4070 Super, 3droad)": 53.0, "CupyNngpr (RTX 4070 Super, 3droad)": 69.0},
+    (200_000, 32): {"TorchNngpr (RTX 4070 Super, 3droad)": 53.0,
+                    "CupyNngpr (RTX 4070 Super, 3droad)": 69.0},
+    (200_000, 64): {"TorchNngpr (RTX 4070 Super, 3droad)": 307.0,
+                    "CupyNngpr (RTX 4070 Super, 3droad)": 272.0},
 }
 
 
-def fit_predict_restarts(ds, n_values=(200_000, 434_874), restarts=3):
+def fit_predict_restarts(ds, cases=((200_000, 32), (434_874, 32), (200_000, 64)), restarts=3):
     """The reference's published benchmark protocol (benchmark.ipynb:115, 186-231,
     427-431) on the road-mixture data: fit(x_train[:n]) with
     ConstantKernel(0.1) * RBF(0.1, ARD over the 3 features) + WhiteKernel(0.1),
-    alpha=1e-8, num_nn=32, n_restarts_optimizer=3 (1 + 3 L-BFGS-B runs), then
+    alpha=1e-8, num_nn=m (32, 64), n_restarts_optimizer=3 (1 + 3 L-BFGS-B runs), then
     predict the test set; the wall time divided by the 3 restarts is the quantity the
     reference plots (seconds per restart)."""
     import numpy as np
     from nngpr_amd import TorchNngpr
     from nngpr_amd import batched_kernels as bk
     out = []
-    for n in n_values:
+    for n, m in cases:
         kern = bk.ConstantKernel(0.1) * bk.RBF(np.full(3, 0.1)) + bk.WhiteKernel(0.1)
-        est = TorchNngpr("cuda:0", kernel=kern, num_nn=32, alpha=1e-8,
+        est = TorchNngpr("cuda:0", kernel=kern, num_nn=m, alpha=1e-8,
                          n_restarts_optimizer=restarts, random_state=0)
         calls = {"n": 0}
         lml = est.log_marginal_likelihood
@@ -146,8 +149,8 @@ def fit_predict_restarts(ds, n_values=(200_000, 434_874), restarts=3):
         t1 = time.perf_counter()
         est.predict(ds["x_test"], return_std=True)
         t_pred = time.perf_counter() - t1
-        ref = REFERENCE_SECONDS_PER_RESTART.get(n)
-        out.append({"n_train": n, "n_test": int(ds["x_test"].shape[0]),
+        ref = REFERENCE_SECONDS_PER_RESTART.get((n, m))
+        out.append({"n_train": n, "num_nn": m, "n_test": int(ds["x_test"].shape[0]),
                     "seconds_per_restart": (t_fit + t_pred) / restarts,
                     "reference_seconds_per_restart": ref,
                     "fit_seconds": t_fit, "predict_seconds": t_pred,

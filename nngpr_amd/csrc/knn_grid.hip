@@ -44,6 +44,10 @@ constexpr int kPointsPerCell = 2;
 constexpr int kScanBlock = 1024;   // items per block of the exclusive scan (256 x 4)
 constexpr int kCoopWaves = 4;      // waves per search block
 constexpr int kCoopSlots = 16;     // LDS append slots per lane
+// Searches whose full grid (the top band's) is at most this many cells wide in x
+// take the instantiation that walks only the boundary rows of rings wider than the
+// x extent.
+constexpr int kNarrowCells = 16;
 
 __device__ __forceinline__ bool point_finite(const double* u, int d) {
     bool fin = true;
@@ -537,7 +541,10 @@ __device__ __forceinline__ void merge_slots(double (&keys)[K], int (&ids)[K],
 // sorted register lists in lock-step (the single merge site of the loop).  After a
 // ring the lane stops once its k-th key is below the squared distance from its
 // query to the nearest cell outside the visited block.
-template <int D, int K>
+// Two instantiations share each launch set: NARROW = false serves searches whose
+// full grid is wider than kNarrowCells in x, NARROW = true the others (every wave of
+// the instantiation that does not serve the search returns at once).
+template <int D, int K, bool NARROW>
 __global__ __launch_bounds__(64 * kCoopWaves)
 // (K = 64: one wave per SIMD, so that the lists live in VGPRs and AGPRs instead of
 // spilling to scratch; 16 % faster on the Vecchia search, 40 % on a query search.)
@@ -557,6 +564,9 @@ void lane_knn_kernel(SearchArgs a) {
     for (int t = 1; t < a.n_bands; ++t)
         if (a.band[t].wave_off <= wg) bj = t;
     const BandDesc& bd0 = a.band[bj];
+    if ((__builtin_amdgcn_readfirstlane(a.band[a.n_bands - 1].gp->g[0]) <= kNarrowCells) !=
+        NARROW)
+        return;
     const int64_t j = (wg - bd0.wave_off) * 64 + lane;
     const bool jv = j < bd0.n_q;
     double qu[3] = {0.0, 0.0, 0.0};
@@ -613,6 +623,12 @@ void lane_knn_kernel(SearchArgs a) {
         // per-lane walk: ring R, segment sg of the ring (two per row), refs [p, e)
         int R = 0, sg = 0, nseg = 0, ny = 1;
         int x0 = 0, x1 = 0, y0 = 0, z0 = 0;
+        // The rows of ring R are every row of its (y, z) block, or - when both end
+        // cells of an interior row lie off the grid (the ring is wider than the x
+        // extent, as in a grid one cell wide in x) - only the block's boundary rows,
+        // whose whole runs are then the ring's only cells: a bottom and a top row
+        // (z = c_z -+ R, all y) and the side rows y = c_y -+ R of the z in between.
+        bool edge = false;
         auto ring_bounds = [&]() {
             x0 = max(c[0] - R, 0);
             x1 = min(c[0] + R, g[0] - 1);
@@ -621,7 +637,36 @@ void lane_knn_kernel(SearchArgs a) {
             z0 = max(c[2] - R, 0);
             const int z1 = min(c[2] + R, g[2] - 1);
             ny = y1 - y0 + 1;
-            nseg = 2 * ny * (z1 - z0 + 1);
+            edge = NARROW && R > 0 && c[0] - R < 0 && c[0] + R > g[0] - 1;
+            if (edge) {
+                const int nbt = (c[2] - R >= 0 ? ny : 0) + (c[2] + R <= g[2] - 1 ? ny : 0);
+                const int nzm = max(min(c[2] + R - 1, g[2] - 1) - max(c[2] - R + 1, 0) + 1, 0);
+                const int ne = (int)(c[1] - R >= 0) + (int)(c[1] + R <= g[1] - 1);
+                nseg = 2 * (nbt + nzm * ne);
+            } else {
+                nseg = 2 * ny * (z1 - z0 + 1);
+            }
+        };
+        // (y, z) of row r of the current ring in the boundary-row enumeration
+        auto edge_row = [&](int r, int& y, int& z) {
+            const int nbot = c[2] - R >= 0 ? ny : 0;
+            if (r < nbot) {
+                y = y0 + r;
+                z = c[2] - R;
+                return;
+            }
+            r -= nbot;
+            const int ntop = c[2] + R <= g[2] - 1 ? ny : 0;
+            if (r < ntop) {
+                y = y0 + r;
+                z = c[2] + R;
+                return;
+            }
+            r -= ntop;
+            const bool ylo = c[1] - R >= 0, yhi = c[1] + R <= g[1] - 1;
+            const bool two = ylo && yhi;
+            z = max(c[2] - R + 1, 0) + (two ? (r >> 1) : r);
+            y = (ylo && !(two && (r & 1))) ? c[1] - R : c[1] + R;
         };
         ring_bounds();
         const uint64_t* rmask = bd0.rowmask;
@@ -793,19 +838,21 @@ void lane_knn_kernel(SearchArgs a) {
                 // occupancy; the first run with a ref is loaded, the empty ones
                 // before it skipped.
                 const int r0 = sg >> 1;
-                const int rz0 = r0 / ny;
+                const int rz0 = edge ? 0 : r0 / ny;
                 const int ry0 = r0 - rz0 * ny;
                 int ra[4], rb[4], rrow[4];
                 bool has[4];
 #pragma unroll
                 for (int v = 0; v < 2; ++v) {
+                    const bool rv = 2 * (r0 + v) < nseg;
                     int yy = ry0 + v, zz = rz0;
                     if (yy >= ny) {
                         yy -= ny;
                         ++zz;
                     }
-                    const bool rv = 2 * (r0 + v) < nseg;
-                    const int y = y0 + yy, z = z0 + zz;
+                    int y = y0 + yy, z = z0 + zz;
+                    if constexpr (NARROW)
+                        if (edge && rv) edge_row(r0 + v, y, z);
                     int a0 = 1, b0 = 0, a1 = 1, b1 = 0;
                     if (rv) row_runs(y, z, a0, b0, a1, b1);
                     const int row = y + g[1] * z;
@@ -917,7 +964,10 @@ __global__ __launch_bounds__(1024) void wave_sort_kernel(const int32_t* cost, in
 
 template <int D, int K>
 void launch_search_dk(const SearchArgs& c, unsigned blocks, hipStream_t st) {
-    hipLaunchKernelGGL((lane_knn_kernel<D, K>), dim3(blocks), dim3(64 * kCoopWaves), 0, st, c);
+    hipLaunchKernelGGL((lane_knn_kernel<D, K, false>), dim3(blocks), dim3(64 * kCoopWaves), 0, st,
+                       c);
+    hipLaunchKernelGGL((lane_knn_kernel<D, K, true>), dim3(blocks), dim3(64 * kCoopWaves), 0, st,
+                       c);
 }
 
 // One launch over every band's waves.
