@@ -181,3 +181,25 @@ def test_methods_agree_on_road_mixture_prefix():
     for r in rows:
         e = vo.knn(x[:r], x[r:r + 1], 0.1, 32)[0] if r > 0 else -np.ones(32, np.int32)
         np.testing.assert_array_equal(outs[0][r], e)
+
+
+@pytest.mark.parametrize("ls", [[1e5, 1e5, 1e-5], [731.42, 302.17, 0.012312]])
+def test_strongly_anisotropic_scales(ls):
+    """Length scales that put the scaled points on (nearly) a line: the grid is one
+    cell wide in x and the search walks only each ring's boundary rows.  Exact
+    against the oracle on a prefix, and bounded in time at 50,000 rows (the full
+    ring walk took seconds there)."""
+    import time
+    ls = np.array(ls)
+    x = np.random.default_rng(7).random((50_000, 3))
+    sub = x[:4000]
+    np.testing.assert_array_equal(hip_knn_self(sub, ls, 32, 0, 0), vo.knn(sub, sub, ls, 32, ref_cut=0))
+    np.testing.assert_array_equal(hip_knn(x, sub[:500], ls, 32, None, 1),
+                                  vo.knn(x, sub[:500], ls, 32))
+    xt = t(x)
+    hip_ops.knn(xt, xt, ls, 32, ref_cut=0, precision=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hip_ops.knn(xt, xt, ls, 32, ref_cut=0, precision=1)
+    torch.cuda.synchronize()
+    assert time.perf_counter() - t0 < 0.5
