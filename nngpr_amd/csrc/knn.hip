@@ -171,7 +171,6 @@ struct ScanArgs {
     int has_cut;
     int64_t cut;
     int reverse;         // process the longest rows first
-    int64_t row_off;     // exact: row index of query 0 (its limit and output row)
     int k_out;           // exact: neighbours written per row (<= K)
     int32_t* out_idx;    // exact: n_query x k_out ; screen: n_query x K candidates
     float* out_thr;      // screen: K-th candidate score per row
@@ -238,7 +237,7 @@ __global__ __launch_bounds__(64) void scan_kernel(ScanArgs<T> a) {
     const bool qv = q < a.n_query;
     int64_t lim64 = a.n_ref;
     if (a.has_cut) {
-        lim64 = q + a.row_off + a.cut;
+        lim64 = q + a.cut;
         lim64 = lim64 < 0 ? 0 : (lim64 > a.n_ref ? a.n_ref : lim64);
     }
     const int32_t lim = qv ? (int32_t)lim64 : 0;
@@ -411,7 +410,7 @@ __global__ __launch_bounds__(64) void scan_kernel(ScanArgs<T> a) {
     } else {
 #pragma unroll
         for (int t = 0; t < KE; ++t)
-            if (t < a.k_out) a.out_idx[(q + a.row_off) * a.k_out + t] = ids[t];
+            if (t < a.k_out) a.out_idx[q * a.k_out + t] = ids[t];
     }
 }
 
