@@ -566,14 +566,49 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
             emit(o, mu, a0, a1);
         }
     } else {
-        // FP32 form: refined solves against the FP64 entries
-        solve(k0, k1, bv0, bv1);
+        // FP32 form: refined solves against the FP64 entries; k and the first target
+        // column go through the factor together (each column's arithmetic is the
+        // one-column solve's), the other columns one at a time
+        double av0, av1;
+        {
+            T u0[2] = {(T)k0, (T)yn0}, u1[2] = {(T)k1, (T)yn1};
+            seg_fwd<T, MR, 2>(L0, L1, dv0, dv1, u0, u1, lane);
+            seg_bwd<T, MR, 2>(L0, L1, dv0, dv1, u0, u1, t);
+            bv0 = (double)u0[0];
+            bv1 = (double)u1[0];
+            av0 = (double)u0[1];
+            av1 = (double)u1[1];
+            xv[i0] = bv0;
+            xv[i1] = bv1;
+            vbuf[i0] = av0;
+            vbuf[i1] = av1;
+            __syncthreads();
+            double qk0 = k0, qk1 = k1, qy0 = yn0, qy1 = yn1;
+#pragma unroll 1
+            for (int j = 0; j < MR; ++j) {
+                const double xj = xv[j], yj = vbuf[j];
+                const double K0 = j == i0 ? kd0 : (j < i0 ? Kt[stri(i0, j)] : Kt[stri(j, i0)]);
+                const double K1 = j == i1 ? kd1 : (j < i1 ? Kt[stri(i1, j)] : Kt[stri(j, i1)]);
+                qk0 = fma(-K0, xj, qk0);
+                qk1 = fma(-K1, xj, qk1);
+                qy0 = fma(-K0, yj, qy0);
+                qy1 = fma(-K1, yj, qy1);
+            }
+            __syncthreads();
+            T w0[2] = {(T)qk0, (T)qy0}, w1[2] = {(T)qk1, (T)qy1};
+            seg_fwd<T, MR, 2>(L0, L1, dv0, dv1, w0, w1, lane);
+            seg_bwd<T, MR, 2>(L0, L1, dv0, dv1, w0, w1, t);
+            bv0 += (double)w0[0];
+            bv1 += (double)w1[0];
+            av0 += (double)w0[1];
+            av1 += (double)w1[1];
+        }
         s2 = c - seg_sum<S>(k0 * bv0 + k1 * bv1);
         sig = sqrt(s2);
-        for (int o = 0; o < ny; ++o) {
-            const double y0 = o == 0 ? yn0 : (v0 ? a.y[(int64_t)j0 * ny + o] : 0.0);
-            const double y1 = o == 0 ? yn1 : (v1 ? a.y[(int64_t)j1 * ny + o] : 0.0);
-            double av0, av1;
+        emit(0, seg_sum<S>(k0 * av0 + k1 * av1), av0, av1);
+        for (int o = 1; o < ny; ++o) {
+            const double y0 = v0 ? a.y[(int64_t)j0 * ny + o] : 0.0;
+            const double y1 = v1 ? a.y[(int64_t)j1 * ny + o] : 0.0;
             solve(y0, y1, av0, av1);
             emit(o, seg_sum<S>(k0 * av0 + k1 * av1), av0, av1);
         }
