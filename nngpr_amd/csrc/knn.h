@@ -105,7 +105,7 @@ struct KnnRefs {
     const double* u64;
     const double* s64;
     const float* s32;
-    const long long* mm;
+    const double* centre;        // screen centre (mean of the finite scaled refs)
     const unsigned int* smax;
     GridRefMem grid;
     int use_grid;

@@ -48,45 +48,62 @@ __device__ __forceinline__ double scaled(const PrepArgs& a, int64_t i, int f) {
     return v;
 }
 
-__global__ void init_minmax_kernel(long long* mm, int d, unsigned int* smax) {
+constexpr int kCentreBlocks = 256;
+
+__global__ void init_ref_kernel(double* centre, int d, unsigned int* smax) {
     int f = threadIdx.x;
-    if (f < d) {
-        mm[2 * f] = 0x7fffffffffffffffLL;
-        mm[2 * f + 1] = (long long)0x8000000000000000ULL;
-    }
+    if (f < d) centre[f] = 0.0;
     if (f == 0) *smax = 0u;
 }
 
-__global__ void minmax_kernel(PrepArgs a, long long* mm) {
-    // one thread per point, block-level pre-reduction through LDS
-    __shared__ long long smin[256], smaxv[256];
+// Centre of the FP32 screen: per feature, the mean of the finite scaled
+// coordinates.  The mean (unlike the box midpoint) moves by only D / n for a point
+// at distance D, so a few far points leave the centred coordinates of the others
+// small.  Each block writes (sum of v / n, finite count) per feature; the final
+// kernel adds the block partials in block order, so the centre is deterministic.
+__global__ __launch_bounds__(256) void centre_partial_kernel(PrepArgs a, double* part) {
+    __shared__ double ssum[256], scnt[256];
+    const double inv_n = 1.0 / (double)a.n;
     for (int f = 0; f < a.d; ++f) {
-        long long lo = 0x7fffffffffffffffLL, hi = (long long)0x8000000000000000ULL;
+        double sum = 0.0, cnt = 0.0;
         for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n;
              i += (int64_t)gridDim.x * blockDim.x) {
-            double v = scaled(a, i, f);
-            if (__builtin_isfinite(v)) {  // the centre ignores NaN and infinite coordinates
-                long long k = dbl_key(v);
-                lo = k < lo ? k : lo;
-                hi = k > hi ? k : hi;
+            const double v = scaled(a, i, f);
+            if (__builtin_isfinite(v)) {  // NaN and infinite coordinates do not enter
+                sum += v * inv_n;
+                cnt += 1.0;
             }
         }
-        smin[threadIdx.x] = lo;
-        smaxv[threadIdx.x] = hi;
+        ssum[threadIdx.x] = sum;
+        scnt[threadIdx.x] = cnt;
         __syncthreads();
-        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-            if (threadIdx.x < s) {
-                smin[threadIdx.x] = min(smin[threadIdx.x], smin[threadIdx.x + s]);
-                smaxv[threadIdx.x] = max(smaxv[threadIdx.x], smaxv[threadIdx.x + s]);
+        for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+            if (threadIdx.x < st) {
+                ssum[threadIdx.x] += ssum[threadIdx.x + st];
+                scnt[threadIdx.x] += scnt[threadIdx.x + st];
             }
             __syncthreads();
         }
         if (threadIdx.x == 0) {
-            atomicMin(&mm[2 * f], smin[0]);
-            atomicMax(&mm[2 * f + 1], smaxv[0]);
+            part[(blockIdx.x * a.d + f) * 2] = ssum[0];
+            part[(blockIdx.x * a.d + f) * 2 + 1] = scnt[0];
         }
         __syncthreads();
     }
+}
+
+__global__ void centre_final_kernel(const double* part, int nb, int d, int64_t n,
+                                    double* centre) {
+    const int f = threadIdx.x;
+    if (f >= d) return;
+    double sum = 0.0, cnt = 0.0;
+    for (int b = 0; b < nb; ++b) {
+        sum += part[(b * d + f) * 2];
+        cnt += part[(b * d + f) * 2 + 1];
+    }
+    // sum = (sum of the finite v) / n: no overflow for finite v
+    const double c = cnt > 0.0 ? sum * ((double)n / cnt) : 0.0;
+    centre[f] = __builtin_isfinite(c) ? c : 0.0;
 }
 
 // Index of feature f of point i in the blocked layout used by the scans: points
@@ -104,7 +121,7 @@ __host__ __device__ __forceinline__ int64_t blk8(int64_t i, int f, int nv) {
 //   p2   (n, optional): |u - centre|^2 in FP64
 //   smax (optional): max_i |u_i - centre| over these points (atomic, float bits)
 // Features d .. dp-1 are zero: they leave every key unchanged.
-__global__ void prep_kernel(PrepArgs a, const long long* mm, int64_t npad, int dp, double* u64,
+__global__ void prep_kernel(PrepArgs a, const double* centre, int64_t npad, int dp, double* u64,
                             double* s64, float* s32, double* p2, unsigned int* smax) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npad) return;
@@ -126,8 +143,7 @@ __global__ void prep_kernel(PrepArgs a, const long long* mm, int64_t npad, int d
         if (u64) u64[i * a.d + f] = v;
         if (s64) s64[blk8(i, f, dp)] = v;
         if (s32) {
-            double c = 0.5 * (key_dbl(mm[2 * f]) + key_dbl(mm[2 * f + 1]));
-            double p = v - c;
+            double p = v - centre[f];
             float pf = (float)p;
             s32[blk8(i, f, dp + 1)] = pf;
             nrm += (double)pf * (double)pf;
@@ -147,13 +163,12 @@ __global__ void prep_kernel(PrepArgs a, const long long* mm, int64_t npad, int d
 }
 
 // |u_i - centre|^2 in FP64 for points whose scaled coordinates are already in u64.
-__global__ void p2_kernel(const double* u64, int64_t n, int d, const long long* mm, double* p2) {
+__global__ void p2_kernel(const double* u64, int64_t n, int d, const double* centre, double* p2) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double pp = 0.0;
     for (int f = 0; f < d; ++f) {
-        double c = 0.5 * (key_dbl(mm[2 * f]) + key_dbl(mm[2 * f + 1]));
-        double p = u64[i * d + f] - c;
+        double p = u64[i * d + f] - centre[f];
         pp += p * p;
     }
     p2[i] = pp;
@@ -478,12 +493,40 @@ __global__ __launch_bounds__(256) void rerank_kernel(RerankArgs a) {
     }
     for (int j = nfin; j < a.k; ++j) a.out_idx[q * a.k + j] = -1;
     if (count < K) return;  // every ref of the row was a candidate: exact
-    // Certificate: every non-candidate r had FP32 score >= thr, so its true key is
-    // >= thr + |p|^2 - E.  Strictly above the m-th candidate key => exact top-m.
+    // Certificate.  A non-candidate r, of centred norm rho (the larger of its exact
+    // and FP32-rounded norms, rho <= S, the largest over the refs), had FP32 score
+    // >= thr, so its exact key is (the FP32 rounding of the screen's score and of
+    // its threshold comparison, E = eps_factor)
+    //   >= f(rho) = thr + |p|^2 - (E + 1e-12) (|thr| + (|p| + rho)^2),
+    // and by the reverse triangle inequality (the exact norm is >= rho / 1.0002)
+    //   >= g(rho) = (rho / 1.0002 - |p|)^2 - 1e-12 (|p| + rho)^2  for rho >= 1.0002 |p|.
+    // f decreases, and g increases on [1.0003 |p|, inf), so for any rc in
+    // [1.0003 |p|, S) every non-candidate key is >= min(f(rc), g(rc)) (rc is taken
+    // at the crossing of f and g); if the crossing or 1.0003 |p| is >= S it is >= f(S).  Far refs then
+    // raise S without loosening the bound of the rows near the centre.  Strictly
+    // above the m-th candidate key => exact top-m.
     const double s = (double)__uint_as_float(*a.smax_bits);
-    const double p = sqrt(a.qp2[q]);
-    const double mag = (p + s) * (p + s);
-    const double bound = (double)a.thr[q] + a.qp2[q] - a.eps_factor * mag - 1e-12 * mag;
+    const double p2 = a.qp2[q];
+    const double p = sqrt(p2);
+    const double cc = a.eps_factor + 1e-12;
+    const double A = (double)a.thr[q] + p2 - cc * fabs((double)a.thr[q]);
+    const double rlo = 1.0003 * p;
+    double bound;
+    if (!(s > rlo)) {
+        bound = A - cc * (p + s) * (p + s);
+    } else {
+        // f(r) = (r - p)^2  <=>  (1 + cc) r^2 - 2 p (1 - cc) r + (1 + cc) p^2 - A = 0
+        const double disc = (1.0 + cc) * A - 4.0 * cc * p2;
+        double rc = disc > 0.0 ? (p * (1.0 - cc) + sqrt(disc)) / (1.0 + cc) : rlo;
+        rc = fmax(rc, rlo);
+        if (!(rc < s)) {  // every ref has rho <= rc: f alone
+            bound = A - cc * (p + s) * (p + s);
+        } else {
+            const double t = rc / 1.0002 - p;
+            const double m2 = (p + rc) * (p + rc);
+            bound = fmin(A - cc * m2, t * t - 1e-12 * m2);
+        }
+    }
     if (!(dm < bound)) {
         unsigned long long slot = atomicAdd(a.flag_count, 1ULL);
         a.flag_rows[slot] = q;
@@ -795,7 +838,8 @@ int launch_rerank(const RerankArgs& a, int K, hipStream_t st) {
 namespace {
 
 struct RefMem {
-    long long* mm;
+    double* centre;
+    double* cpart;
     unsigned int* smax;
     double* u64;
     double* s64;
@@ -809,7 +853,8 @@ RefMem carve_refs(Carver& c, int64_t n_ref, int d, int precision) {
     RefMem M{};
     const int64_t rpad = round_up(std::max<int64_t>(n_ref, 1), 64) + 64;
     const int dp = scan_dim(d);
-    M.mm = c.take<long long>(2 * NNGP_MAX_DIM);
+    M.centre = c.take<double>(NNGP_MAX_DIM);
+    M.cpart = c.take<double>((size_t)kCentreBlocks * 2 * NNGP_MAX_DIM);
     M.smax = c.take<unsigned int>(4);
     M.u64 = c.take<double>((size_t)n_ref * d + 1);
     if (d <= 32) {
@@ -913,7 +958,7 @@ int knn_prepare_refs(const double* ref, int64_t n_ref, int d, const double* scal
     R.s32 = M.s32;
     R.grid = M.grid;
     R.use_grid = grid ? 1 : 0;
-    R.mm = M.mm;
+    R.centre = M.centre;
     R.smax = M.smax;
     R.n = n_ref;
     R.d = d;
@@ -921,7 +966,7 @@ int knn_prepare_refs(const double* ref, int64_t n_ref, int d, const double* scal
     R.n_scale = n_scale;
     for (int f = 0; f < n_scale; ++f) R.scale[f] = scale[f];
     *out = R;
-    hipLaunchKernelGGL(init_minmax_kernel, dim3(1), dim3(64), 0, st, M.mm, d, M.smax);
+    hipLaunchKernelGGL(init_ref_kernel, dim3(1), dim3(64), 0, st, M.centre, d, M.smax);
     if (n_ref == 0) {
         NNGP_HIP_CHECK(hipGetLastError());
         return NNGP_OK;
@@ -932,10 +977,15 @@ int knn_prepare_refs(const double* ref, int64_t n_ref, int d, const double* scal
     pr.d = d;
     pr.n_scale = n_scale;
     for (int f = 0; f < n_scale; ++f) pr.scale[f] = scale[f];
-    if (M.s32) hipLaunchKernelGGL(minmax_kernel, dim3(256), dim3(256), 0, st, pr, M.mm);
+    if (M.s32) {
+        hipLaunchKernelGGL(centre_partial_kernel, dim3(kCentreBlocks), dim3(256), 0, st, pr,
+                           M.cpart);
+        hipLaunchKernelGGL(centre_final_kernel, dim3(1), dim3(64), 0, st, M.cpart, kCentreBlocks,
+                           d, n_ref, M.centre);
+    }
     const int64_t rpad = round_up(n_ref, 64) + 64;
     hipLaunchKernelGGL(prep_kernel, dim3((unsigned)((rpad + 255) / 256)), dim3(256), 0, st, pr,
-                       M.mm, rpad, scan_dim(d), M.u64, M.s64, M.s32, (double*)nullptr, M.smax);
+                       M.centre, rpad, scan_dim(d), M.u64, M.s64, M.s32, (double*)nullptr, M.smax);
     NNGP_HIP_CHECK(hipGetLastError());
     if (grid) return grid_build(M.u64, n_ref, d, M.grid, st);
     return NNGP_OK;
@@ -984,7 +1034,7 @@ int knn_search(const KnnRefs& R, const double* query, int64_t n_query, int k, in
         double* qs64w = fast && !screen ? Q.s64 : nullptr;
         float* qs32w = screen ? Q.s32 : nullptr;
         hipLaunchKernelGGL(prep_kernel, dim3((unsigned)((qpad + 255) / 256)), dim3(256), 0, st, pr,
-                           R.mm, qpad, dp, Q.u64, qs64w, qs32w, screen ? Q.p2 : nullptr,
+                           R.centre, qpad, dp, Q.u64, qs64w, qs32w, screen ? Q.p2 : nullptr,
                            (unsigned int*)nullptr);
         qu64 = Q.u64;
         qs64 = qs64w;
@@ -993,7 +1043,7 @@ int knn_search(const KnnRefs& R, const double* query, int64_t n_query, int k, in
         // |u_q - centre|^2 of the refs themselves (the query set is the ref set)
         pr.x = nullptr;
         hipLaunchKernelGGL(p2_kernel, dim3((unsigned)((n_query + 255) / 256)), dim3(256), 0, st,
-                           R.u64, n_query, d, R.mm, Q.p2);
+                           R.u64, n_query, d, R.centre, Q.p2);
     }
     NNGP_HIP_CHECK(hipGetLastError());
 

@@ -203,3 +203,60 @@ def test_strongly_anisotropic_scales(ls):
     hip_ops.knn(xt, xt, ls, 32, ref_cut=0, precision=1)
     torch.cuda.synchronize()
     assert time.perf_counter() - t0 < 0.5
+
+
+@pytest.mark.parametrize("cut", [None, 0])
+@pytest.mark.parametrize("case", ["one_far_point", "far_cluster", "heavy_tails"])
+def test_screen_with_outliers(case, cut):
+    """FP32 screen (brute-force scans, d = 5) on point sets with far points: the
+    indices stay exact, and the per-row certificate (centred on the mean, bounded
+    with the row's own norm) still certifies almost every row, so far points do not
+    send the whole search to the exact per-row recomputation."""
+    rng = np.random.default_rng(["one_far_point", "far_cluster", "heavy_tails"].index(case))
+    n, d, k = 6000, 5, 16
+    x = rng.random((n, d))
+    if case == "one_far_point":
+        x[n // 2] = 1000.0
+    elif case == "far_cluster":
+        x[n // 3: n // 3 + n // 100] = x[n // 3: n // 3 + n // 100] * 0.01 + 100.0
+    else:
+        x = rng.standard_t(2.0, size=(n, d))
+    with hip_ops.knn_method(hip_ops.KNN_BRUTE_FORCE):
+        got = hip_knn_self(x, 0.1, k, cut, 1)
+        rec = hip_ops.knn_stats()["knn_rows_recomputed"]
+    np.testing.assert_array_equal(got, vo.knn(x, x, 0.1, k, ref_cut=cut))
+    # rows whose query is itself far from every other point keep a wide margin
+    # requirement; the rest certify
+    limit = {"one_far_point": 2, "far_cluster": n // 100 + 2, "heavy_tails": n // 50}[case]
+    assert rec <= limit, rec
+
+
+@pytest.mark.parametrize("cut", [None, 0])
+@pytest.mark.parametrize("case", ["one_far_point", "few_far_points", "far_cluster", "heavy_tails"])
+def test_grid_with_outliers(case, cut):
+    """Uniform-grid search (d = 3) when a few points lie far out (the box they
+    stretch leaves most refs in a few cells): indices stay exact for the far rows,
+    the rows around them, and separate queries far outside on every side."""
+    rng = np.random.default_rng(10 + ["one_far_point", "few_far_points", "far_cluster",
+                                      "heavy_tails"].index(case))
+    n, k = 5000, 16
+    x = rng.random((n, 3))
+    if case == "one_far_point":
+        x[n // 2] = [1000.0, -1000.0, 3.0]
+    elif case == "few_far_points":
+        far = rng.choice(n, 6, replace=False)
+        x[far] = rng.normal(0.0, 50.0, (6, 3))
+        x[far[0], 1] = 0.5                      # far in x and z only
+        x[far[1]] = x[far[2]] + [1e-3, 0, 0]    # two far points next to each other
+    elif case == "far_cluster":
+        x[n // 3: n // 3 + 50] = x[n // 3: n // 3 + 50] * 0.01 + 100.0
+    else:
+        x = rng.standard_t(1.5, size=(n, 3))
+    with hip_ops.knn_method(hip_ops.KNN_AUTO):
+        got = hip_knn_self(x, 0.1, k, cut, 1)
+        np.testing.assert_array_equal(got, vo.knn(x, x, 0.1, k, ref_cut=cut))
+        if cut is None:
+            xq = np.concatenate([rng.random((200, 3)), rng.normal(0.0, 300.0, (100, 3)),
+                                 x[rng.choice(n, 50)] + 1e-4])
+            np.testing.assert_array_equal(hip_knn(x, xq, 0.1, k, None, 0),
+                                          vo.knn(x, xq, 0.1, k))
