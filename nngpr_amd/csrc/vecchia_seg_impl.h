@@ -634,9 +634,16 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
         // Length-scale term of the pair (xi, X[j]) with dk / dK weight W and
         // stationary value E: W G against the squared distance (accumulated
         // without the factor 2 h_f, applied at the end).  RBF: W G = WE = W E.
-        auto add_dist = [&](const double (&xi)[DM], int j, double W, double E, double WE) {
+        // ard_c: std::true_type / std::false_type (narrow rows: the pair loops
+        // carry no branch on it) or a run-time bool (wide rows: one copy of the
+        // loops keeps the register budget).
+        auto add_dist = [&](auto ard_c, const double (&xi)[DM], int j, double W, double E,
+                            double WE) {
+            bool is_ard;
+            if constexpr (std::is_same_v<decltype(ard_c), bool>) is_ard = ard_c;
+            else is_ard = decltype(ard_c)::value;
             const double* xj = X + j * DM;
-            if (!ard) {
+            if (!is_ard) {
                 double u = 0.0;
 #pragma unroll
                 for (int f = 0; f < DM; ++f) {
@@ -666,7 +673,7 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
         // Row i: the self term, the diagonal (W_ii = G_s b_i^2 - b_i a_i, with
         // dK_ii free of the length scales) and the off-diagonal pairs j < i with
         // W_ij doubled: 2 G_s b_i b_j - b_i a_j - a_i b_j = c1 b_j - b_i a_j.
-        auto row_adj = [&](auto ncols, int i, const double* krow, bool vi, double bi,
+        auto row_adj = [&](auto ard_c, auto ncols, int i, const double* krow, bool vi, double bi,
                            double ati, double eis) {
             constexpr int N = decltype(ncols)::value;
             double xi[DM];
@@ -676,28 +683,44 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
                 const double v = vi ? ati - 2.0 * Gs * bi : 0.0;
                 acc_C += v;
                 acc_s += v * eis;
-                add_dist(xi, MR, v, eis, v * eis);
+                add_dist(ard_c, xi, MR, v, eis, v * eis);
                 const double wd = vi ? (Gs * bi - ati) * bi : 0.0;
                 acc_w += wd;
                 acc_C += wd;
                 acc_s += wd;
             }
             const double c1 = 2.0 * Gs * bi - ati;
-#pragma unroll 2
+            // wide rows unroll less: the loads of unrolled iterations are hoisted
+            constexpr int UA = DM <= 3 ? 4 : (DM <= 8 ? 2 : 1);
+#pragma unroll UA
             for (int j = 0; j < N; ++j) {
                 const bool ok = vi & (j < i) & row_valid<MR>(j, vlo, vhi);
-                const double bj = xv[j], aj = vbuf[j];
+                // b_j, a_j and the entry are loaded unconditionally and pinned in
+                // registers, so that the mask is a select after the loads rather
+                // than a branch around them.  For j >= i the entry read is the
+                // next row's (for the last row of the last segment: the first
+                // vector after the triangles), inside the block's LDS, and the
+                // select discards it.
+                double bj = xv[j], aj = vbuf[j], kij = krow[j];
+                asm volatile("" : "+v"(bj), "+v"(aj), "+v"(kij));
                 const double W = ok ? fma(c1, bj, -bi * aj) : 0.0;
-                // entries right of the diagonal are other rows' data: never used
-                const double E = ok ? e_of(krow[j]) : 0.0;
+                const double E = ok ? e_of(kij) : 0.0;
                 const double WE = W * E;
                 acc_C += W;
                 acc_s += WE;
-                add_dist(xi, j, W, E, WE);
+                add_dist(ard_c, xi, j, W, E, WE);
             }
         };
-        row_adj(std::integral_constant<int, S>{}, i0, krow0, v0, bv0, at0, e0s);
-        row_adj(std::integral_constant<int, MR>{}, i1, krow1, v1, bv1, at1, e1s);
+        auto adjoint_rows = [&](auto ard_c) {
+            row_adj(ard_c, std::integral_constant<int, S>{}, i0, krow0, v0, bv0, at0, e0s);
+            row_adj(ard_c, std::integral_constant<int, MR>{}, i1, krow1, v1, bv1, at1, e1s);
+        };
+        if constexpr (DM <= 8) {
+            if (ard) adjoint_rows(std::true_type{});
+            else adjoint_rows(std::false_type{});
+        } else {
+            adjoint_rows(ard);
+        }
         acc_l *= 2.0 * kp.h[0];
 #pragma unroll
         for (int f = 0; f < DM; ++f) accf[f] *= 2.0 * hw[f];
