@@ -318,11 +318,16 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
     double hw[DM];
 #pragma unroll
     for (int f = 0; f < DM; ++f) hw[f] = (ard && f < d) ? kp.h[f] : 0.0;
-    // scaled squared distance between row coordinates xi (registers or LDS) and X[j]
-    auto sqdist = [&](const auto& xi, int j) -> double {
+    // scaled squared distance between row coordinates xi (registers or LDS) and X[j];
+    // ard_c: std::true_type / std::false_type (narrow rows: no branch in the
+    // entry loop) or a run-time bool (wide rows)
+    auto sqdist = [&](auto ard_c, const auto& xi, int j) -> double {
+        bool is_ard;
+        if constexpr (std::is_same_v<decltype(ard_c), bool>) is_ard = ard_c;
+        else is_ard = decltype(ard_c)::value;
         const double* xj = X + j * DM;
         double u = 0.0;
-        if (!ard) {
+        if (!is_ard) {
 #pragma unroll
             for (int f = 0; f < DM; ++f) {
                 const double tt = xi[f] - xj[f];
@@ -345,36 +350,44 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
     // stationary part E of a valid off-diagonal pair from its covariance entry
     auto e_of = [&](double K) -> double { return kp.n_ls > 0 ? (K - kp.C) * inv_s : 0.0; };
     double e0s = 0.0, e1s = 0.0;
+    auto fill_entries = [&](auto ard_c) {
 #pragma unroll 1
-    for (int e0 = 0; e0 < NE; e0 += U) {
+        for (int e0 = 0; e0 < NE; e0 += U) {
 #pragma unroll
-        for (int uu = 0; uu < U; ++uu) {
-            const int e = e0 + uu;
-            if (e < NE) {
-                const SegEntry s = seg_entry<MR>(e, i0, i1);
-                const bool vi = s.first ? v0 : v1;
-                const bool vj = s.j == MR ? true : row_valid<MR>(s.j, vlo, vhi);
-                double u;
-                if constexpr (DM <= 8) {
-                    double xi[DM];
+            for (int uu = 0; uu < U; ++uu) {
+                const int e = e0 + uu;
+                if (e < NE) {
+                    const SegEntry s = seg_entry<MR>(e, i0, i1);
+                    const bool vi = s.first ? v0 : v1;
+                    const bool vj = s.j == MR ? true : row_valid<MR>(s.j, vlo, vhi);
+                    double u;
+                    if constexpr (DM <= 8) {
+                        double xi[DM];
 #pragma unroll
-                    for (int f = 0; f < DM; ++f) xi[f] = s.first ? xr0[f] : xr1[f];
-                    u = sqdist(xi, s.j);
-                } else {  // wide rows: the lane's coordinates are read back from LDS
-                    u = sqdist(X + s.i * DM, s.j);
-                }
-                const double Ev = (vi && vj && kp.n_ls > 0 && s.i != s.j)
-                                      ? (MAT ? cov_e(kp.fam, u) : exp(-u))
-                                      : (s.i == s.j ? 1.0 : 0.0);
-                if (e <= MR) {
-                    if (s.j < s.i) Kt[stri(s.i, s.j)] = (vi && vj) ? fma(kp.s, Ev, kp.C) : 0.0;
-                } else if (e == MR + 1) {
-                    e0s = Ev;
-                } else {
-                    e1s = Ev;
+                        for (int f = 0; f < DM; ++f) xi[f] = s.first ? xr0[f] : xr1[f];
+                        u = sqdist(ard_c, xi, s.j);
+                    } else {  // wide rows: the lane's coordinates are read back from LDS
+                        u = sqdist(ard_c, X + s.i * DM, s.j);
+                    }
+                    const double Ev = (vi && vj && kp.n_ls > 0 && s.i != s.j)
+                                          ? (MAT ? cov_e(kp.fam, u) : exp(-u))
+                                          : (s.i == s.j ? 1.0 : 0.0);
+                    if (e <= MR) {
+                        if (s.j < s.i) Kt[stri(s.i, s.j)] = (vi && vj) ? fma(kp.s, Ev, kp.C) : 0.0;
+                    } else if (e == MR + 1) {
+                        e0s = Ev;
+                    } else {
+                        e1s = Ev;
+                    }
                 }
             }
         }
+    };
+    if constexpr (DM <= 8) {
+        if (ard) fill_entries(std::true_type{});
+        else fill_entries(std::false_type{});
+    } else {
+        fill_entries(ard);
     }
     const double k0 = v0 ? fma(kp.s, e0s, kp.C) : 0.0;
     const double k1 = v1 ? fma(kp.s, e1s, kp.C) : 0.0;
