@@ -189,6 +189,12 @@ struct ScanArgs {
     int k_out;           // exact: neighbours written per row (<= K)
     int32_t* out_idx;    // exact: n_query x k_out ; screen: n_query x K candidates
     float* out_thr;      // screen: K-th candidate score per row
+    // screen with gridDim.y > 1 ref stripes: stripe s scans refs
+    // [s * stripe_w, (s + 1) * stripe_w) (the last stripe to the end) and writes
+    // its sorted packed list to out_pk[(s * n_query + q) * K ...] instead of
+    // out_idx / out_thr; stripe_merge_kernel merges the stripes
+    int32_t stripe_w;    // multiple of 8
+    uint64_t* out_pk;
 };
 
 
@@ -250,15 +256,21 @@ __global__ __launch_bounds__(64) void scan_kernel(ScanArgs<T> a) {
     const int64_t blk = a.reverse ? (int64_t)(gridDim.x - 1 - blockIdx.x) : (int64_t)blockIdx.x;
     const int64_t q = blk * 64 + lane;
     const bool qv = q < a.n_query;
+    // ref stripe [rs, re) of this block (the whole range with one stripe)
+    const bool striped = gridDim.y > 1;
+    const int32_t rs = striped ? (int32_t)blockIdx.y * a.stripe_w : 0;
+    const int64_t re = (striped && blockIdx.y + 1 < gridDim.y) ? (int64_t)rs + a.stripe_w
+                                                                : (int64_t)a.n_ref;
     int64_t lim64 = a.n_ref;
     if (a.has_cut) {
         lim64 = q + a.cut;
         lim64 = lim64 < 0 ? 0 : (lim64 > a.n_ref ? a.n_ref : lim64);
     }
+    lim64 = lim64 < re ? lim64 : re;
     const int32_t lim = qv ? (int32_t)lim64 : 0;
-    const int32_t lim_lo = wave_min32(qv ? lim : a.n_ref);
+    const int32_t lim_lo = wave_min32(qv ? lim : (int32_t)re);
     const int32_t lim_hi = wave_max32(lim);
-    const int32_t full_end = lim_lo & ~7;  // bulk region, whole 8-ref blocks
+    const int32_t full_end = lim_lo & ~7;  // bulk region [rs, full_end), whole 8-ref blocks
 
     T qr[D];
     const int64_t qc = qv ? q : 0;
@@ -399,8 +411,8 @@ __global__ __launch_bounds__(64) void scan_kernel(ScanArgs<T> a) {
     };
     {
         T ra[NV][SUB], rb[NV][SUB];
-        load_sub(ra, a.ref);
-        int32_t r0 = 0;
+        load_sub(ra, sub_ptr(rs));
+        int32_t r0 = rs;
         while (r0 < full_end) {
             load_sub(rb, sub_ptr(r0 + SUB));
             score_sub(ra, r0);
@@ -412,21 +424,64 @@ __global__ __launch_bounds__(64) void scan_kernel(ScanArgs<T> a) {
         }
     }
     // Tail: refs past some lanes' Vecchia limit.
-    for (int32_t r0 = full_end & ~7; r0 < lim_hi; r0 += 8) {
+    for (int32_t r0 = full_end > rs ? full_end : rs; r0 < lim_hi; r0 += 8) {
         const T* bp = a.ref + (int64_t)(r0 >> 3) * (NV * 8);
         take_block(bp, r0, 8, true);
     }
     if (__any(cnt > 0)) merge();
     if (!qv) return;
     if constexpr (SCREEN) {
+        if (striped) {
+            uint64_t* o = a.out_pk + ((int64_t)blockIdx.y * a.n_query + q) * K;
 #pragma unroll
-        for (int t = 0; t < K; ++t) a.out_idx[q * K + t] = (int)(uint32_t)pk[t];
-        a.out_thr[q] = unord_f32((uint32_t)(pk[K - 1] >> 32));
+            for (int t = 0; t < K; ++t) o[t] = pk[t];
+        } else {
+#pragma unroll
+            for (int t = 0; t < K; ++t) a.out_idx[q * K + t] = (int)(uint32_t)pk[t];
+            a.out_thr[q] = unord_f32((uint32_t)(pk[K - 1] >> 32));
+        }
     } else {
 #pragma unroll
         for (int t = 0; t < KE; ++t)
             if (t < a.k_out) a.out_idx[q * a.k_out + t] = ids[t];
     }
+}
+
+// Merge of the ref stripes' screen lists: per row, the K smallest packed
+// (score, index) values of the stripes' sorted lists, as candidates and the K-th
+// score.  Each stripe list holds its stripe's K smallest, so the merge holds the
+// K smallest over all refs, and every ref left out scored at least the merged
+// K-th value: the certificate of the unstriped screen applies unchanged.
+constexpr int kMaxStripes = 8;
+template <int K>
+__global__ __launch_bounds__(256) void stripe_merge_kernel(const uint64_t* pk, int n_stripes,
+                                                           int64_t n_query, int32_t* cand,
+                                                           float* thr) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n_query) return;
+    int h[kMaxStripes];
+#pragma unroll
+    for (int s = 0; s < kMaxStripes; ++s) h[s] = 0;
+    uint64_t last = 0;
+    for (int t = 0; t < K; ++t) {
+        uint64_t best = ~0ull;
+        int bs = 0;
+#pragma unroll
+        for (int s = 0; s < kMaxStripes; ++s) {
+            if (s < n_stripes && h[s] < K) {
+                const uint64_t v = pk[((int64_t)s * n_query + q) * K + h[s]];
+                if (v < best) {
+                    best = v;
+                    bs = s;
+                }
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < kMaxStripes; ++s) h[s] += s == bs ? 1 : 0;
+        cand[q * K + t] = (int)(uint32_t)best;
+        last = best;
+    }
+    thr[q] = unord_f32((uint32_t)(last >> 32));
 }
 
 // ---------------------------------------------------------------------------
@@ -758,6 +813,19 @@ __global__ __launch_bounds__(64) void prefix_knn_kernel(const double* ru64, cons
 // Rows whose margin turns out too thin fail the certificate and are redone.
 inline int screen_K(int k) { return (int)round_up(k + 8, 8); }
 
+// Ref stripes of the screen scan: with few query blocks (one lane per query) the
+// scan cannot fill the chip, so each block's refs are split over up to 8 stripes
+// (blocks) of at least 2,048 refs, for about 2,048 blocks in all.
+inline int screen_stripes(int64_t n_ref, int64_t n_query) {
+    const int64_t nblk = std::max<int64_t>((n_query + 63) / 64, 1);
+    int64_t s = std::min<int64_t>(2048 / nblk, kMaxStripes);
+    s = std::min<int64_t>(s, n_ref / 2048);
+    return (int)std::max<int64_t>(s, 1);
+}
+inline int32_t screen_stripe_w(int64_t n_ref, int n_stripes) {
+    return (int32_t)round_up((n_ref + n_stripes - 1) / n_stripes, 8);
+}
+
 bool use_fast_scan(int d, int k) { return d >= 1 && d <= 32 && k >= 1 && k <= 64; }
 
 // Feature count the scan kernels are compiled for; extra features are zero in
@@ -771,7 +839,14 @@ template <typename T, int D, int K, bool SCREEN>
 void launch_scan_u(const ScanArgs<T>& a, hipStream_t st) {
     constexpr int B = SCREEN ? 32 : 16;
     const int64_t nblk = (a.n_query + 63) / 64;
-    hipLaunchKernelGGL((scan_kernel<T, D, K, B, SCREEN>), dim3((unsigned)nblk), dim3(64), 0, st, a);
+    const unsigned ns = (SCREEN && a.out_pk) ? (unsigned)((a.n_ref + a.stripe_w - 1) / a.stripe_w) : 1u;
+    hipLaunchKernelGGL((scan_kernel<T, D, K, B, SCREEN>), dim3((unsigned)nblk, ns), dim3(64), 0, st,
+                       a);
+    if constexpr (SCREEN) {
+        if (ns > 1)
+            hipLaunchKernelGGL((stripe_merge_kernel<K>), dim3((unsigned)((a.n_query + 255) / 256)),
+                               dim3(256), 0, st, a.out_pk, (int)ns, a.n_query, a.out_idx, a.out_thr);
+    }
 }
 
 template <typename T, int D, bool SCREEN>
@@ -872,6 +947,7 @@ struct QueryMem {
     double* p2;
     int32_t* cand;
     float* thr;
+    uint64_t* spk;    // per-stripe screen lists (more than one stripe)
     int64_t* flag_rows;
     unsigned long long* flag_count;
     double* flag_key;
@@ -900,6 +976,10 @@ QueryMem carve_query(Carver& c, int64_t n_ref, int64_t n_query, int d, int k, in
         const int K = screen_K(k);
         Q.cand = c.take<int32_t>((size_t)n_query * K + 1);
         Q.thr = c.take<float>((size_t)n_query + 1);
+        // stripe lists: stripes x rows <= min(8 n_query, 2048 x 64) for every
+        // n_query' <= n_query, so a workspace sized for a batch serves smaller ones
+        if (n_ref >= 2 * 2048)
+            Q.spk = c.take<uint64_t>((size_t)std::min<int64_t>(8 * n_query, 2048 * 64) * K + 1);
         Q.flag_rows = c.take<int64_t>((size_t)n_query + 1);
         Q.flag_key = c.take<double>((size_t)n_query + 1);
         Q.flag_idx = c.take<int32_t>((size_t)n_query + 1);
@@ -1117,6 +1197,11 @@ int knn_search(const KnnRefs& R, const double* query, int64_t n_query, int k, in
     a.reverse = has_cut;
     a.out_idx = Q.cand;
     a.out_thr = Q.thr;
+    const int n_stripes = screen_stripes(R.n, n_query);
+    if (n_stripes > 1 && Q.spk) {
+        a.stripe_w = screen_stripe_w(R.n, n_stripes);
+        a.out_pk = Q.spk;
+    }
     hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, Q.flag_count, 1);
     hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, Q.over_count, 1);
     timing_begin(TK_KNN_SCAN, st);
