@@ -193,8 +193,13 @@ struct ScanArgs {
     // [s * stripe_w, (s + 1) * stripe_w) (the last stripe to the end) and writes
     // its sorted packed list to out_pk[(s * n_query + q) * K ...] instead of
     // out_idx / out_thr; stripe_merge_kernel merges the stripes
-    int32_t stripe_w;    // multiple of 8
+    int32_t stripe_w;    // multiple of 8 (0: one stripe)
     uint64_t* out_pk;
+    // exact scan with ref stripes: stripe s writes its sorted (key, index) list
+    // to out_skey / out_sidx[(s * n_query + q) * K ...]; stripe_merge_exact_kernel
+    // writes the k_out smallest pairs
+    T* out_skey;
+    int32_t* out_sidx;
 };
 
 
@@ -441,9 +446,18 @@ __global__ __launch_bounds__(64) void scan_kernel(ScanArgs<T> a) {
             a.out_thr[q] = unord_f32((uint32_t)(pk[K - 1] >> 32));
         }
     } else {
+        if (striped) {
+            const int64_t o = ((int64_t)blockIdx.y * a.n_query + q) * K;
 #pragma unroll
-        for (int t = 0; t < KE; ++t)
-            if (t < a.k_out) a.out_idx[q * a.k_out + t] = ids[t];
+            for (int t = 0; t < KE; ++t) {
+                a.out_skey[o + t] = keys[t];
+                a.out_sidx[o + t] = ids[t];
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < KE; ++t)
+                if (t < a.k_out) a.out_idx[q * a.k_out + t] = ids[t];
+        }
     }
 }
 
@@ -482,6 +496,41 @@ __global__ __launch_bounds__(256) void stripe_merge_kernel(const uint64_t* pk, i
         last = best;
     }
     thr[q] = unord_f32((uint32_t)(last >> 32));
+}
+
+// Merge of the exact scan's stripe lists: per row, the k_out smallest (key,
+// index) pairs of the stripes' sorted lists (empty slots (+inf, -1) last).
+template <int K>
+__global__ __launch_bounds__(256) void stripe_merge_exact_kernel(const double* sk,
+                                                                 const int32_t* si,
+                                                                 int n_stripes, int64_t n_query,
+                                                                 int k_out, int32_t* out_idx) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n_query) return;
+    int h[kMaxStripes];
+#pragma unroll
+    for (int s = 0; s < kMaxStripes; ++s) h[s] = 0;
+    for (int t = 0; t < k_out; ++t) {
+        double bk = __builtin_inf();
+        uint32_t bi = 0xFFFFFFFFu;
+        int bs = -1;
+#pragma unroll
+        for (int s = 0; s < kMaxStripes; ++s) {
+            if (s < n_stripes && h[s] < K) {
+                const int64_t o = ((int64_t)s * n_query + q) * K + h[s];
+                const double kk = sk[o];
+                const uint32_t ii = (uint32_t)si[o];
+                if (bs < 0 || kk < bk || (kk == bk && ii < bi)) {
+                    bk = kk;
+                    bi = ii;
+                    bs = s;
+                }
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < kMaxStripes; ++s) h[s] += s == bs ? 1 : 0;
+        out_idx[q * k_out + t] = (int32_t)bi;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -813,8 +862,8 @@ __global__ __launch_bounds__(64) void prefix_knn_kernel(const double* ru64, cons
 // Rows whose margin turns out too thin fail the certificate and are redone.
 inline int screen_K(int k) { return (int)round_up(k + 8, 8); }
 
-// Ref stripes of the screen scan: with few query blocks (one lane per query) the
-// scan cannot fill the chip, so each block's refs are split over up to 8 stripes
+// Ref stripes of the scans (screen and exact): with few query blocks (one lane per
+// query) a scan cannot fill the chip, so each block's refs are split over up to 8 stripes
 // (blocks) of at least 2,048 refs, for about 2,048 blocks in all.
 inline int screen_stripes(int64_t n_ref, int64_t n_query) {
     const int64_t nblk = std::max<int64_t>((n_query + 63) / 64, 1);
@@ -839,13 +888,17 @@ template <typename T, int D, int K, bool SCREEN>
 void launch_scan_u(const ScanArgs<T>& a, hipStream_t st) {
     constexpr int B = SCREEN ? 32 : 16;
     const int64_t nblk = (a.n_query + 63) / 64;
-    const unsigned ns = (SCREEN && a.out_pk) ? (unsigned)((a.n_ref + a.stripe_w - 1) / a.stripe_w) : 1u;
+    const unsigned ns = a.stripe_w > 0 ? (unsigned)((a.n_ref + a.stripe_w - 1) / a.stripe_w) : 1u;
     hipLaunchKernelGGL((scan_kernel<T, D, K, B, SCREEN>), dim3((unsigned)nblk, ns), dim3(64), 0, st,
                        a);
-    if constexpr (SCREEN) {
-        if (ns > 1)
-            hipLaunchKernelGGL((stripe_merge_kernel<K>), dim3((unsigned)((a.n_query + 255) / 256)),
-                               dim3(256), 0, st, a.out_pk, (int)ns, a.n_query, a.out_idx, a.out_thr);
+    if (ns > 1) {
+        const dim3 mg((unsigned)((a.n_query + 255) / 256));
+        if constexpr (SCREEN)
+            hipLaunchKernelGGL((stripe_merge_kernel<K>), mg, dim3(256), 0, st, a.out_pk, (int)ns,
+                               a.n_query, a.out_idx, a.out_thr);
+        else
+            hipLaunchKernelGGL((stripe_merge_exact_kernel<K>), mg, dim3(256), 0, st, a.out_skey,
+                               a.out_sidx, (int)ns, a.n_query, a.k_out, a.out_idx);
     }
 }
 
@@ -948,6 +1001,8 @@ struct QueryMem {
     int32_t* cand;
     float* thr;
     uint64_t* spk;    // per-stripe screen lists (more than one stripe)
+    double* sxk;      // per-stripe exact-scan lists: keys
+    int32_t* sxi;     //   and indices
     int64_t* flag_rows;
     unsigned long long* flag_count;
     double* flag_key;
@@ -970,6 +1025,11 @@ QueryMem carve_query(Carver& c, int64_t n_ref, int64_t n_query, int d, int k, in
         Q.u64 = c.take<double>((size_t)n_query * d + 1);
         if (fast && !screen) Q.s64 = c.take<double>((size_t)qpad * dp);
         if (screen) Q.s32 = c.take<float>((size_t)qpad * (dp + 1));
+    }
+    if (fast && !screen && n_ref >= 2 * 2048) {  // exact-scan stripe lists (bound as below)
+        const size_t ne = (size_t)std::min<int64_t>(8 * n_query, 2048 * 64) * exact_K(k) + 1;
+        Q.sxk = c.take<double>(ne);
+        Q.sxi = c.take<int32_t>(ne);
     }
     if (screen) {
         Q.p2 = c.take<double>((size_t)n_query + 1);
@@ -1178,6 +1238,12 @@ int knn_search(const KnnRefs& R, const double* query, int64_t n_query, int k, in
         a.cut = ref_cut;
         a.reverse = has_cut;
         a.k_out = k;
+        const int n_stripes = screen_stripes(R.n, n_query);
+        if (n_stripes > 1 && Q.sxk) {
+            a.stripe_w = screen_stripe_w(R.n, n_stripes);
+            a.out_skey = Q.sxk;
+            a.out_sidx = Q.sxi;
+        }
         a.out_idx = out_idx;
         timing_begin(TK_KNN_SCAN, st);
         const int rc = launch_scan<double, false>(a, d, exact_K(k), st);
