@@ -263,7 +263,7 @@ def test_grid_with_outliers(case, cut):
 
 
 @pytest.mark.parametrize("prec", [0, 1])
-@pytest.mark.parametrize("cut", [None, 0])
+@pytest.mark.parametrize("cut", [None, 0, 2, -1])
 def test_ref_stripes_small_inputs(prec, cut):
     """Few query blocks (n = 9,000, d = 18): the scans split the refs into stripes and
     merge the stripes' lists; the indices equal the oracle's, including the first
