@@ -139,7 +139,8 @@ int nngp_knn_get_method(void);
  *   out_k (n_points x width x width FP64, row-major), out_grad (n_points x width
  *   x width x n_theta FP64, or NULL).
  * The fused entry points below never materialise these blocks; this one exists
- * for the plug-in surface and for tests.
+ * for the plug-in surface and for tests.  A slot outside [-1, n_x) is never
+ * read through: its row and column of out_k (and of out_grad) are NaN.
  */
 int nngp_fill_nn_kernel(const double* x, int64_t n_x, int d, const int32_t* nn,
                         int64_t n_points, int width, const nngp_kernel* kernel,
@@ -156,6 +157,9 @@ int nngp_fill_nn_kernel(const double* x, int64_t n_x, int d, const int32_t* nn,
  *   out_grad (device, n_theta doubles, or NULL when eval_gradient == 0).
  *   out_point_loglik (device, row1 - row0 doubles, or NULL).
  * Sums are reduced in a fixed order: results are bitwise reproducible.
+ * Neighbour indices outside [-1, n) are read as empty slots (no gather leaves x
+ * and y) and counted; nngp_read_stats on the workspace then returns
+ * NNGP_ERR_INVALID_ARGUMENT.  The same holds for nngp_predict with n_train.
  */
 size_t nngp_vecchia_workspace_bytes(int64_t n_rows, int m, int d, int ny, int n_theta,
                                     int precision, int eval_gradient);
@@ -261,7 +265,10 @@ int nngp_model_predict(const nngp_model* model, const double* x_query, int64_t n
  * Counters of the last nngp_knn / nngp_vecchia_loglik / nngp_predict call that
  * used this workspace: rows whose kNN certificate failed and were recomputed
  * exactly, points re-factored in FP64 after the FP32 pivot check.  Synchronises
- * the stream.  No reference counterpart (diagnostics of the new precision mode).
+ * the stream.  Returns NNGP_ERR_INVALID_ARGUMENT when that call met neighbour
+ * indices outside [-1, n), else NNGP_ERR_NOT_POSITIVE_DEFINITE when a block
+ * could not be factored, else NNGP_OK.  No reference counterpart (diagnostics of
+ * the new precision mode).
  */
 typedef struct {
     int64_t knn_rows_recomputed;

@@ -3,10 +3,11 @@
 `vecchia_log_likelihood(theta, x, y, nn, kernel, alpha)` returns the Vecchia
 log-likelihood of the training rows as a 0-d tensor that is differentiable with
 respect to `theta`, the log-hyperparameters of `kernel` (scikit-learn theta order,
-as in GaussianProcessRegressor.log_marginal_likelihood).  Forward runs the fused
-fill -> Cholesky -> mu/sigma -> loglik kernels only; backward runs the fused pass
-again with the hand-written adjoint, which recomputes the neighbour factors in
-registers instead of storing them, and returns dl/dtheta.
+as in GaussianProcessRegressor.log_marginal_likelihood).  Forward runs one fused
+fill -> Cholesky -> mu/sigma -> loglik pass; when theta requires a gradient that
+same pass also runs the hand-written adjoint (which recomputes the neighbour
+factors in registers instead of storing them) and keeps the nθ-vector dl/dtheta,
+so backward only scales it by the incoming gradient.
 
 This is the gradient of base_nngpr.py:586-614 (same quantity, computed by the
 closed-form adjoint rather than K_gradient solves).
@@ -31,21 +32,17 @@ class VecchiaLogLikelihood(torch.autograd.Function):
     def forward(ctx, theta, x, y, nn, kernel, alpha, precision):
         th = theta.detach().cpu().numpy().astype(np.float64)
         ks = _kernel_struct(kernel, th)
-        ll, _, _ = hip_ops.vecchia_loglik(x, y, nn, ks, alpha, precision=precision,
-                                          eval_gradient=False)
-        ctx.save_for_backward(theta)
-        ctx.inputs = (x, y, nn, kernel, alpha, precision)
+        # one fused pass: the adjoint gradient comes with the value when theta
+        # needs it, and backward only scales it
+        need = bool(ctx.needs_input_grad[0])
+        ll, g, _ = hip_ops.vecchia_loglik(x, y, nn, ks, alpha, precision=precision,
+                                          eval_gradient=need)
+        ctx.grad = g.to(theta.device, theta.dtype) if need else None
         return ll[0].to(theta.device)
 
     @staticmethod
     def backward(ctx, gout):
-        (theta,) = ctx.saved_tensors
-        x, y, nn, kernel, alpha, precision = ctx.inputs
-        th = theta.detach().cpu().numpy().astype(np.float64)
-        ks = _kernel_struct(kernel, th)
-        _, g, _ = hip_ops.vecchia_loglik(x, y, nn, ks, alpha, precision=precision,
-                                         eval_gradient=True)
-        return (gout * g.to(theta.device, theta.dtype), None, None, None, None, None, None)
+        return (gout * ctx.grad, None, None, None, None, None, None)
 
 
 def vecchia_log_likelihood(theta, x, y, nn, kernel, alpha=1e-10,

@@ -19,6 +19,7 @@ from operator import itemgetter
 
 import numpy as np
 import torch
+import xxhash
 from sklearn import base as sk_base
 from sklearn.gaussian_process import GaussianProcessRegressor
 from sklearn.preprocessing._data import _handle_zeros_in_scale
@@ -51,6 +52,18 @@ def _knn_lists(ref, query, scale, k, ref_cut=None, precision=hip_ops.PRECISION_F
                                   "candidate row (%d here)" % (nat.NNGP_MAX_K, int(avail.max())))
     idx = torch.arange(k, device=dev)[None, :]
     return torch.where(idx < avail[:, None], idx, -1).to(torch.int32).contiguous()
+
+
+def _fingerprint(a):
+    """Content hash of a host array (shape, dtype and bytes)."""
+    a = np.ascontiguousarray(a)
+    h = xxhash.xxh3_64()
+    h.update(repr((a.shape, a.dtype.str)).encode())
+    try:
+        h.update(memoryview(a).cast("B"))
+    except (TypeError, ValueError):  # buffers memoryview cannot view as bytes
+        h.update(a.tobytes())
+    return h.intdigest()
 
 
 def _precisions(allow_downcast_f32):
@@ -131,10 +144,15 @@ class HipWorker:
                            ref_cut=ref_cut_index, precision=precision)
 
     def fill_nn_kernel(self, x, nn_indices, kernel, eval_gradient=False):
-        """base_nngpr.py:1781-1826 semantics on the HIP fill kernel."""
+        """base_nngpr.py:1781-1826 semantics on the HIP fill kernel.  An index
+        outside [-1, len(x)) raises IndexError, as the generic path's gather
+        x[nn_indices] does."""
         ks, _ = self.kernel_struct(kernel)
-        return hip_ops.fill_nn_kernel(self.xp.from_numpy(x), self.xp.from_numpy(nn_indices), ks,
-                                      eval_gradient=eval_gradient)
+        x = self.xp.from_numpy(x)
+        nn = self.xp.from_numpy(nn_indices)
+        if nn.numel() and bool(((nn < -1) | (nn >= x.shape[0])).any()):
+            raise IndexError("neighbour index out of range for %d points" % x.shape[0])
+        return hip_ops.fill_nn_kernel(x, nn, ks, eval_gradient=eval_gradient)
 
     def batched_chofactor(self, x, may_overwrite_x=False):
         return torch.linalg.cholesky(x, upper=False), True
@@ -231,8 +249,16 @@ class Nngpr(GaussianProcessRegressor):
         return state
 
     def _device_train(self):
-        """Training arrays resident on the device (uploaded once per fit)."""
+        """Training arrays resident on the device (uploaded once per fit).
+
+        With copy_X_train=False, X_train_ / y_train_ are the caller's arrays and
+        may change in place after fit (scikit-learn's documented behaviour: the
+        predictions follow them); outside fit's optimizer loop the cache is then
+        keyed on a fingerprint of their contents too, so a change re-uploads the
+        data and drops the cached neighbour lists."""
         key = (id(self.X_train_), id(self.y_train_))
+        if not self.copy_X_train and not getattr(self, "_optimizing", False):
+            key += (_fingerprint(self.X_train_), _fingerprint(self.y_train_))
         cache = getattr(self, "_dev_cache", None)
         if cache is None or cache[0] != key:
             xp = self._worker.get_array_module()
@@ -290,7 +316,20 @@ class Nngpr(GaussianProcessRegressor):
                 raise ValueError("alpha must be a scalar or an array with only one element")
         self.X_train_ = np.copy(X) if self.copy_X_train else X
         self.y_train_ = np.copy(y) if self.copy_X_train else y
+        # a new fit never reuses an earlier upload (the arrays may be the same
+        # objects with new contents)
+        self._dev_cache = None
+        self._optimizing = True
+        try:
+            self._fit_hyperparameters()
+        finally:
+            self._optimizing = False
+        for w in self.workers:
+            w.clear_cache()
+        return self
 
+    def _fit_hyperparameters(self):
+        """L-BFGS-B restarts of fit (base_nngpr.py:327-383) on the uploaded data."""
         if self.optimizer is not None and self.kernel_.n_dims > 0:
             def obj_func(theta, eval_gradient=True):
                 if eval_gradient:
@@ -316,9 +355,6 @@ class Nngpr(GaussianProcessRegressor):
         else:
             self.log_marginal_likelihood_value_ = self.log_marginal_likelihood(
                 self.kernel_.theta, clone_kernel=False)
-        for w in self.workers:
-            w.clear_cache()
-        return self
 
     # ------------------------------------------------------------ log-likelihood
     def log_marginal_likelihood(self, theta=None, eval_gradient=False, clone_kernel=True):

@@ -282,13 +282,12 @@ int nngp_knn_get_method(void) { return knn_method(); }
 int nngp_fill_nn_kernel(const double* x, int64_t n_x, int d, const int32_t* nn,
                         int64_t n_points, int width, const nngp_kernel* kernel, double* out_k,
                         double* out_grad, void* stream) {
-    (void)n_x;
-    if (d < 1 || d > NNGP_MAX_DIM || width < 1 || n_points < 0 || !out_k || !nn || !x)
+    if (d < 1 || d > NNGP_MAX_DIM || width < 1 || n_points < 0 || n_x < 0 || !out_k || !nn || !x)
         return NNGP_ERR_INVALID_ARGUMENT;
     KernParams kp;
     int rc = to_kern_params(kernel, d, &kp);
     if (rc) return rc;
-    launch_fill(x, d, nn, n_points, width, kp, out_k, out_grad, (hipStream_t)stream);
+    launch_fill(x, n_x, d, nn, n_points, width, kp, out_k, out_grad, (hipStream_t)stream);
     NNGP_HIP_CHECK(hipGetLastError());
     return NNGP_OK;
 }
@@ -329,6 +328,7 @@ int nngp_vecchia_loglik(const double* x, const double* y, int64_t n, int d, int 
     a.xs = x + row0 * d;
     a.ys = y + row0 * ny;
     a.nn = nn + row0 * m;
+    a.n_src = n;
     a.npts = npts;
     a.d = d;
     a.ny = ny;
@@ -364,9 +364,8 @@ int nngp_predict(const double* x_train, const double* y_train, int64_t n_train, 
                  const nngp_kernel* kernel, double alpha, int precision, double* out_mean,
                  double* out_std, void* workspace, size_t workspace_bytes, void* stream) {
     hipStream_t st = (hipStream_t)stream;
-    (void)n_train;
     if (d < 1 || d > NNGP_MAX_DIM || ny < 1 || m < 1 || m > NNGP_MAX_NN || n_query < 0 ||
-        !out_mean)
+        n_train < 0 || !out_mean)
         return NNGP_ERR_INVALID_ARGUMENT;
     if (precision != NNGP_PRECISION_FP64 && precision != NNGP_PRECISION_FP32_REFINED)
         return NNGP_ERR_INVALID_ARGUMENT;
@@ -383,6 +382,7 @@ int nngp_predict(const double* x_train, const double* y_train, int64_t n_train, 
     a.xs = x_query;
     a.ys = nullptr;
     a.nn = nn;
+    a.n_src = n_train;
     a.npts = n_query;
     a.d = d;
     a.ny = ny;
@@ -602,6 +602,7 @@ int nngp_model_predict(const nngp_model* model, const double* x_query, int64_t n
     a.xs = x_query;
     a.ys = nullptr;
     a.nn = nn;
+    a.n_src = M->n;
     a.npts = n_query;
     a.d = M->d;
     a.ny = M->ny;
@@ -627,6 +628,7 @@ int nngp_read_stats(const void* workspace, nngp_stats* out, void* stream) {
     NNGP_HIP_CHECK(hipMemcpy(&h, workspace, sizeof(h), hipMemcpyDeviceToHost));
     out->knn_rows_recomputed = h.knn_rows_recomputed;
     out->points_refactored_fp64 = h.points_refactored_fp64;
+    if (h.bad_index > 0) return NNGP_ERR_INVALID_ARGUMENT;
     return h.not_pd > 0 ? NNGP_ERR_NOT_POSITIVE_DEFINITE : NNGP_OK;
 }
 

@@ -57,8 +57,18 @@ struct WsHeader {
     int64_t knn_rows_recomputed;
     int64_t points_refactored_fp64;
     int64_t not_pd;
-    int64_t reserved[29];
+    int64_t bad_index;  // neighbour slots outside [-1, n_src), read as empty
+    int64_t reserved[28];
 };
+
+// A neighbour slot as the fused kernels use it: a row of the source set
+// [0, n_src) or -1 (empty).  Any other value is counted in hdr->bad_index (when
+// hdr is given) and read as an empty slot, so no gather leaves the source rows.
+__device__ __forceinline__ int nbr_slot(int j, int64_t n_src, WsHeader* hdr) {
+    if (j >= 0 && (int64_t)j < n_src) return j;
+    if (j != -1 && hdr) atomicAdd(reinterpret_cast<unsigned long long*>(&hdr->bad_index), 1ULL);
+    return -1;
+}
 
 static __global__ void zero_header_kernel(WsHeader* h) {
     int i = threadIdx.x;

@@ -351,6 +351,7 @@ int cond_run(const CondProblem& P, void* ws, size_t ws_bytes, hipStream_t st) {
     a.xs = P.x_query;
     a.ys = nullptr;
     a.nn = P.nn;
+    a.n_src = nt + nq;
     a.npts = nq;
     a.d = d;
     a.ny = ny;

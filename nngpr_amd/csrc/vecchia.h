@@ -57,6 +57,7 @@ struct VecArgs {
     const double* xs;      // own coordinates (npts x d)
     const double* ys;      // own targets (npts x ny) for the log-likelihood, else NULL
     const int32_t* nn;     // npts x m neighbour indices into x, -1 padded
+    int64_t n_src;         // rows of x and y (indices outside [-1, n_src) are invalid)
     int64_t npts;
     int d, ny, m;
     double alpha;
@@ -79,7 +80,7 @@ struct VecArgs {
 size_t generic_ws_doubles_per_point(int m);
 int64_t generic_chunk(int64_t npts, int m);
 void launch_generic(const VecArgs& a, double* wsL, double* wsv, int64_t P, hipStream_t st);
-void launch_fill(const double* x, int d, const int32_t* nn, int64_t npts, int width,
+void launch_fill(const double* x, int64_t n_x, int d, const int32_t* nn, int64_t npts, int width,
                  const KernParams& kp, double* outK, double* outG, hipStream_t st);
 size_t reduce_partial_doubles(int cols);
 void launch_reduce(const double* v, int64_t rows, int cols, double* partial, double* out_ll,

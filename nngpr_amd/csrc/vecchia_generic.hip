@@ -61,12 +61,15 @@ __global__ __launch_bounds__(256) void vecchia_generic_kernel(VecArgs a, int64_t
 #define BV(i) wsv[((int64_t)(m + (i))) * P + t]
 #define TV(i) wsv[((int64_t)(2 * m + (i))) * P + t]
 #define AT(i) wsv[((int64_t)(3 * m + (i))) * P + t]
+    // neighbour slot i: a source row or -1 (invalid entries read as empty; the
+    // fill's first read of each slot counts them in the header)
+    auto slot = [&](int i) { return nbr_slot(nnp[i], a.n_src, nullptr); };
     // ---- fill ----
     double Ed, ud;
     for (int i = 0; i < m; ++i) {
-        const int ji = nnp[i];
+        const int ji = nbr_slot(nnp[i], a.n_src, a.hdr);
         for (int j = 0; j <= i; ++j) {
-            const int jj = nnp[j];
+            const int jj = slot(j);
             double v;
             if (ji < 0 || jj < 0) {
                 v = (i == j) ? 1.0 : 0.0;
@@ -133,7 +136,7 @@ __global__ __launch_bounds__(256) void vecchia_generic_kernel(VecArgs a, int64_t
     const double half_log_2pi = 0.91893853320467274178;
     for (int o = 0; o < ny; ++o) {
         for (int i = 0; i < m; ++i) {
-            const int ji = nnp[i];
+            const int ji = slot(i);
             TV(i) = (ji < 0) ? 0.0 : a.y[(int64_t)ji * ny + o];
         }
         double mu = 0.0;
@@ -169,7 +172,7 @@ __global__ __launch_bounds__(256) void vecchia_generic_kernel(VecArgs a, int64_t
     if (ard)
         for (int f = 0; f < d; ++f) acc_lf[f] = 0.0;
     for (int j = 0; j < m; ++j) {
-        const int jj = nnp[j];
+        const int jj = slot(j);
         if (jj < 0) continue;
         const double* xj = a.x + (int64_t)jj * d;
         const double bj = BV(j), aj = AT(j);
@@ -188,7 +191,7 @@ __global__ __launch_bounds__(256) void vecchia_generic_kernel(VecArgs a, int64_t
             }
         // neighbour - neighbour entries (l <= j; off-diagonal counted twice)
         for (int l = 0; l <= j; ++l) {
-            const int jl = nnp[l];
+            const int jl = slot(l);
             if (jl < 0) continue;
             const double bl = BV(l), al = AT(l);
             double W = Gs * bj * bl - 0.5 * (bj * al + aj * bl);
@@ -225,22 +228,30 @@ __global__ __launch_bounds__(256) void vecchia_generic_kernel(VecArgs a, int64_t
 #undef AT
 }
 
-// Materialised neighbour blocks: one thread per (point, row i).
-__global__ __launch_bounds__(256) void fill_kernel(const double* x, int d, const int32_t* nn,
-                                                   int64_t npts, int width, KernParams kp,
-                                                   double* outK, double* outG) {
+// Materialised neighbour blocks: one thread per (point, row i).  A slot outside
+// [-1, n_x) is never read through: its row and column are NaN (K and gradient).
+__global__ __launch_bounds__(256) void fill_kernel(const double* x, int64_t n_x, int d,
+                                                   const int32_t* nn, int64_t npts, int width,
+                                                   KernParams kp, double* outK, double* outG) {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= npts * width) return;
     const int64_t p = gid / width;
     const int i = (int)(gid % width);
-    const int ji = nn[p * width + i];
+    const int ri = nn[p * width + i];
+    const int ji = nbr_slot(ri, n_x, nullptr);
     const int nt = kp.n_theta;
     for (int j = 0; j < width; ++j) {
-        const int jj = nn[p * width + j];
+        const int rj = nn[p * width + j];
+        const int jj = nbr_slot(rj, n_x, nullptr);
         const int64_t e = (p * width + i) * width + j;
         double* g = outG ? outG + e * nt : nullptr;
+        const bool bad = (ji < 0 && ri != -1) || (jj < 0 && rj != -1);
         if (g)
-            for (int q = 0; q < nt; ++q) g[q] = 0.0;
+            for (int q = 0; q < nt; ++q) g[q] = bad ? __builtin_nan("") : 0.0;
+        if (bad) {
+            outK[e] = __builtin_nan("");
+            continue;
+        }
         if (ji < 0 || jj < 0) {
             outK[e] = (i == j) ? 1.0 : 0.0;
             continue;
@@ -328,12 +339,12 @@ void launch_generic(const VecArgs& a, double* wsL, double* wsv, int64_t P, hipSt
     }
 }
 
-void launch_fill(const double* x, int d, const int32_t* nn, int64_t npts, int width,
+void launch_fill(const double* x, int64_t n_x, int d, const int32_t* nn, int64_t npts, int width,
                  const KernParams& kp, double* outK, double* outG, hipStream_t st) {
     const int64_t tot = npts * width;
     if (tot == 0) return;
-    hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, x, d, nn,
-                       npts, width, kp, outK, outG);
+    hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, x, n_x, d,
+                       nn, npts, width, kp, outK, outG);
 }
 
 size_t reduce_partial_doubles(int cols) { return (size_t)kReduceBlocks * cols; }

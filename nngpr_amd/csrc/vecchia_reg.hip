@@ -123,7 +123,7 @@ __global__ __launch_bounds__(64) void vecchia_reg_kernel(VecArgs a, RegExtra ex)
 
         // ---------------- gather --------------------------------------------
         int jr = -1;
-        if (pv && r < a.m) jr = a.nn[p * a.m + r];
+        if (pv && r < a.m) jr = nbr_slot(a.nn[p * a.m + r], a.n_src, a.hdr);
         Ig[r] = jr;
         for (int f = 0; f < d; ++f) Xg[r * d + f] = jr >= 0 ? a.x[(int64_t)jr * d + f] : 0.0;
         for (int f = r; f < d; f += MR) Xg[MR * d + f] = pv ? a.xs[p * d + f] : 0.0;

@@ -283,8 +283,8 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
 #pragma unroll
     for (int f = 0; f < DM; ++f) xs_r[f] = a.xs[p * d + min(f, d - 1)];
     const double ys_r = a.ys ? a.ys[p * ny] : 0.0;
-    const int j0 = (pv && i0 < a.m) ? n0 : -1;
-    const int j1 = (pv && i1 < a.m) ? n1 : -1;
+    const int j0 = (pv && i0 < a.m) ? nbr_slot(n0, a.n_src, a.hdr) : -1;
+    const int j1 = (pv && i1 < a.m) ? nbr_slot(n1, a.n_src, a.hdr) : -1;
     const bool v0 = j0 >= 0, v1 = j1 >= 0;
     const int64_t c0 = v0 ? j0 : 0, c1 = v1 ? j1 : 0;
     double xr0[DM], xr1[DM];

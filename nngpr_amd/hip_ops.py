@@ -104,7 +104,10 @@ def knn(ref, query, scale, k, ref_cut=None, precision=PRECISION_FP32_REFINED, ou
     return out
 
 
-def knn_stats(ws_slot="knn", dev=None):
+def stats(ws_slot, dev=None):
+    """Counters of the last call that used workspace slot `ws_slot` ("knn",
+    "vecchia", "predict", ...): kNN rows recomputed exactly, points re-factored in
+    FP64 by the FP32 form.  None when the slot was never used."""
     dev = dev or torch.device("cuda", torch.cuda.current_device())
     buf = _workspaces.get((str(dev), ws_slot))
     if buf is None:
@@ -113,6 +116,10 @@ def knn_stats(ws_slot="knn", dev=None):
     nat.lib().nngp_read_stats(_p(buf), ctypes.byref(st), _stream(dev))
     return {"knn_rows_recomputed": st.knn_rows_recomputed,
             "points_refactored_fp64": st.points_refactored_fp64}
+
+
+def knn_stats(ws_slot="knn", dev=None):
+    return stats(ws_slot, dev)
 
 
 def fill_nn_kernel(x, nn, kstruct, eval_gradient=False):
@@ -138,6 +145,8 @@ def _check_pd(ws, dev, what):
     if rc == nat.NNGP_ERR_NOT_POSITIVE_DEFINITE:
         raise np.linalg.LinAlgError(
             "%s: a neighbour covariance block is not positive definite" % what)
+    if rc == nat.NNGP_ERR_INVALID_ARGUMENT:
+        raise ValueError("%s: a neighbour index lies outside [-1, number of rows)" % what)
     nat.check(rc, "nngp_read_stats")
     return st
 

@@ -152,6 +152,96 @@ def test_fill_kernel_vs_oracle():
     np.testing.assert_allclose(G.cpu().numpy(), Go, rtol=1e-12, atol=1e-15)
 
 
+def near_duplicate_set(seed=12):
+    """300 points in the unit square, then a copy of each moved by about 1e-3 (rows
+    300-599, so every copy conditions on its original).  With l = 0.05, White(1e-6)
+    and alpha = 0 a block holding both copies has a pivot near 4e-4 against a
+    diagonal of 1 (pivot ratio ~2.5e3 > 1e3), which FP64 still factors to ~1e-12;
+    blocks of the first 300 rows are well conditioned."""
+    rng = np.random.default_rng(seed)
+    base = rng.random((300, 2))
+    twin = base + rng.normal(0, 1e-3 / np.sqrt(2), base.shape)
+    x = np.concatenate([base, twin], axis=0)
+    y = np.sin(4 * x[:, 0]) + np.cos(3 * x[:, 1]) + rng.normal(0, 0.05, 600)
+    return x, y
+
+
+@pytest.mark.parametrize("what", ["loglik", "predict"])
+def test_fp32_pivot_ratio_refactor_matches_oracle(what):
+    """The FP32 form sends blocks whose pivot ratio exceeds 1e3 (or that fail to
+    factor) to an FP64 re-factor in the same call: the count is positive and the
+    results equal the FP64 form and the oracle."""
+    x, y = near_duplicate_set()
+    w, s, ls, alpha, m = 1e-6, 1.0, 0.05, 0.0, 12
+    tm = {"noise_level": 0, "rbf_level": 1, "length_scale": 2}
+    ks = kstruct(w, 0.0, s, ls, tm, 3)
+    if what == "loglik":
+        nn = vo.vecchia_knn(x, ls, m)
+        ll_o, g_o, pts_o = vo.vecchia_loglik(x, y, nn, w, 0.0, s, ls, alpha, tm, 3,
+                                             eval_gradient=True)
+        out = {}
+        for prec in (0, 1):
+            ll, g, pts = hip_ops.vecchia_loglik(t(x), t(y), t(nn, torch.int32), ks, alpha,
+                                                precision=prec, eval_gradient=True,
+                                                point_loglik=True)
+            out[prec] = (ll.item(), g.cpu().numpy(), pts.cpu().numpy(),
+                         hip_ops.stats("vecchia")["points_refactored_fp64"])
+        assert out[0][3] == 0
+        assert 0 < out[1][3] < x.shape[0]
+        for prec, rt in ((0, 1e-8), (1, 1e-6)):
+            assert abs(out[prec][0] - ll_o) / x.shape[0] <= rt
+            np.testing.assert_allclose(out[prec][2], pts_o, rtol=10 * rt, atol=10 * rt)
+            np.testing.assert_allclose(out[prec][1], g_o, rtol=10 * rt, atol=10 * rt)
+    else:
+        xq = x[::7] + 3e-5
+        nn = vo.knn(x, xq, ls, m)
+        mean_o, std_o = vo.predict(x, y, xq, nn, w, 0.0, s, ls, alpha)
+        for prec in (0, 1):
+            mean, std = hip_ops.predict(t(x), t(y), t(xq), t(nn, torch.int32),
+                                        kstruct(w, 0.0, s, ls, {}, 0), alpha, precision=prec)
+            nref = hip_ops.stats("predict")["points_refactored_fp64"]
+            assert (nref == 0) if prec == 0 else (nref > 0)
+            np.testing.assert_allclose(mean.cpu().numpy(), mean_o, rtol=1e-6, atol=1e-7)
+            np.testing.assert_allclose(std.cpu().numpy(), std_o, rtol=1e-5)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_neighbour_index_out_of_range_rejected(prec):
+    """An index >= n (or below -1) is never gathered: the fused entry points report
+    it through the workspace status (ValueError), the plug-in fill raises
+    IndexError, and the raw fill entry point returns NaN for that slot."""
+    rng = np.random.default_rng(5)
+    x = rng.random((200, 3))
+    y = rng.normal(size=200)
+    nn = vo.vecchia_knn(x, 0.3, 8)
+    ks = kstruct(0.05, 0.0, 1.0, 0.3, {"noise_level": 0, "rbf_level": 1, "length_scale": 2}, 3)
+    for bad in (200, 10 ** 9, -7):
+        nb = nn.copy()
+        nb[150, 3] = bad
+        with pytest.raises(ValueError, match="outside"):
+            hip_ops.vecchia_loglik(t(x), t(y), t(nb, torch.int32), ks, 1e-10, precision=prec,
+                                   eval_gradient=True)
+        with pytest.raises(ValueError, match="outside"):
+            hip_ops.predict(t(x), t(y), t(x[:20]), t(nb[140:160], torch.int32),
+                            kstruct(0.05, 0.0, 1.0, 0.3, {}, 0), 1e-10, precision=prec)
+    # the same call with valid indices still succeeds afterwards
+    ll, _, _ = hip_ops.vecchia_loglik(t(x), t(y), t(nn, torch.int32), ks, 1e-10, precision=prec)
+    ll_o, _, _ = vo.vecchia_loglik(x, y, nn, 0.05, 0.0, 1.0, 0.3, 1e-10)
+    assert abs(ll.item() - ll_o) / 200 <= (1e-12 if prec == 0 else 1e-6)
+    nb = nn[:10].copy()
+    nb[4, 2] = 200
+    K, _ = hip_ops.fill_nn_kernel(t(x), t(nb, torch.int32), ks)
+    K = K.cpu().numpy()
+    assert np.isnan(K[4, 2]).all() and np.isnan(K[4, :, 2]).all()
+    assert np.isfinite(np.delete(np.delete(K[4], 2, 0), 2, 1)).all()
+    assert np.isfinite(K[[0, 1, 2, 3, 5]]).all()
+    from nngpr_amd.base_nngpr import HipWorker
+    from nngpr_amd import batched_kernels as bkm
+    kern = bkm.WhiteKernel(0.05) + bkm.RBF(0.3)
+    with pytest.raises(IndexError):
+        HipWorker("cuda:0").fill_nn_kernel(x, nb, kern)
+
+
 @pytest.mark.parametrize("prec", [0, 1])
 def test_not_positive_definite_raises(prec):
     x = np.repeat(np.random.default_rng(1).random((20, 2)), 3, axis=0)

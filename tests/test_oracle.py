@@ -244,6 +244,62 @@ def test_exact_gp_predict():
     np.testing.assert_allclose(std, std_ref, atol=1e-10)
 
 
+def test_exact_gp_predict_alpha_only_on_training_rows():
+    """alpha = 1e-3 is large enough to tell the reference's rule apart: alpha is
+    added to the training neighbours' variances only, never to the query's own
+    K_00 (base_nngpr.py:1106-1107), which is what scikit-learn's exact predictive
+    std does too.  Adding it to K_00 moves std by ~alpha / (2 std) and fails."""
+    x, x_eval, y = ref_test_data()
+    y = y[:, 1]
+    alpha = 1e-3
+    kern = sk_kernel(1.3, 0.8, 0.05)
+    gpr = skgp.GaussianProcessRegressor(kernel=kern, optimizer=None, alpha=alpha).fit(x, y)
+    mean_ref, std_ref = gpr.predict(x_eval, return_std=True)
+    nn = vo.knn(x, x_eval, 0.8, x.shape[0])
+    mean, std = vo.predict(x, y, x_eval, nn, 0.05, 0.0, 1.3, 0.8, alpha)
+    np.testing.assert_allclose(mean[:, 0], mean_ref, rtol=1e-9, atol=1e-10)
+    np.testing.assert_allclose(std, std_ref, rtol=1e-9, atol=1e-10)
+    # planted mistake: alpha on the query's own variance as well
+    std_bad = np.sqrt(std ** 2 + alpha)
+    assert not np.allclose(std_bad, std_ref, rtol=1e-6, atol=1e-8)
+
+
+# ------------------------------------------- numpy_port (the timed CPU baseline)
+def test_numpy_port_knn_matches_oracle():
+    """The restated generic kNN (dense -k blocks + argsort) returns the oracle's
+    neighbour lists on tie-free data, with and without the Vecchia cut."""
+    from oracle import numpy_port as npp
+    rng = np.random.default_rng(31)
+    x = rng.random((400, 3))
+    q = rng.random((90, 3))
+    np.testing.assert_array_equal(npp.generic_knn(x, q, 0.0, 1.0, 0.3, 7, None),
+                                  vo.knn(x, q, 0.3, 7))
+    for i0, i1 in [(0, 50), (120, 400)]:
+        got = npp.generic_knn(x[:i1], x[i0:i1], 0.2, 1.4, 0.3, 7, i0)
+        np.testing.assert_array_equal(got, vo.vecchia_knn(x, 0.3, 7)[i0:i1])
+
+
+@pytest.mark.parametrize("i0,i1", [(0, 40), (40, 300), (250, 400)])
+def test_numpy_port_loglik_matches_oracle(i0, i1):
+    """One task of the restated CPU worker (rows [i0, i1), neighbours among the
+    rows before each) equals the oracle's per-row log-likelihood summed over the
+    same rows, and its reference-formula gradient the oracle's gradient."""
+    from oracle import numpy_port as npp
+    rng = np.random.default_rng(32)
+    x = rng.random((400, 3))
+    y = np.sin(5 * x[:, 0]) + x[:, 1] * x[:, 2] + rng.normal(0, 0.1, 400)
+    w, C, s, ls, k, alpha = 0.02, 0.0, 1.1, 0.25, 9, 1e-8
+    tmap = {"noise_level": 0, "rbf_level": 1, "length_scale": 2}
+    ll, g, _, _ = npp.loglik_batch(x, y, i0, i1, w, C, s, ls, k, alpha, tmap, 3)
+    nn = vo.vecchia_knn(x, ls, k)
+    rows = np.arange(i0, i1)
+    _, _, pts = vo.vecchia_loglik(x, y, nn, w, C, s, ls, alpha, tmap, 3)
+    np.testing.assert_allclose(ll, pts[rows].sum(), rtol=1e-10)
+    # the oracle's gradient of the same rows: its adjoint restricted to them
+    g_o = vo.vecchia_adjoint_grad(x, y, nn, w, C, s, ls, alpha, tmap, 3, rows=rows)
+    np.testing.assert_allclose(g, g_o, rtol=1e-9, atol=1e-10)
+
+
 # ----------------------------------------------------------- Vecchia, m < n, brute
 def test_vecchia_matches_dense_conditionals():
     rng = np.random.default_rng(9)
