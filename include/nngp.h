@@ -171,6 +171,30 @@ int nngp_vecchia_loglik(const double* x, const double* y, int64_t n, int d, int 
                         void* stream);
 
 /*
+ * nngp_vecchia_loglik without a workspace: it allocates nothing (no hipMalloc,
+ * no caller buffer) and may be called from code under HIP graph capture or from
+ * loops that must not allocate.  No counterpart in the reference; it is for
+ * callers that cannot allocate.  Same arguments and results as
+ * nngp_vecchia_loglik minus (workspace, workspace_bytes); its scratch is a
+ * fixed NNGP_NOALLOC_SCRATCH_BYTES array in device memory, one per GPU, held
+ * from the first call on that GPU until the process ends.  Calls whose scratch
+ * need (nngp_vecchia_workspace_bytes of the same arguments: about
+ * (row1 - row0) * (1 + n_theta) * 8 bytes, plus (row1 - row0) * 8 in the FP32
+ * form, for m <= 64) exceeds it return NNGP_ERR_WORKSPACE_TOO_SMALL (detail in
+ * nngp_last_error_detail) before any launch: 434,874 rows with 3 theta fit in
+ * either form.  Every call on a GPU shares that GPU's scratch, so calls that
+ * overlap on one GPU (other streams, other threads) race.  A block that cannot
+ * be factored or a neighbour index outside [-1, n) makes out_loglik (and
+ * out_grad) NaN.
+ */
+#define NNGP_NOALLOC_SCRATCH_BYTES ((size_t)32 << 20)
+int nngp_vecchia_loglik_noalloc(const double* x, const double* y, int64_t n, int d, int ny,
+                                const int32_t* nn, int m, int64_t row0, int64_t row1,
+                                const nngp_kernel* kernel, double alpha, int precision,
+                                int eval_gradient, double* out_loglik, double* out_grad,
+                                double* out_point_loglik, void* stream);
+
+/*
  * Only-train prediction.  Replaces Nngpr._predict_worker (base_nngpr.py:1101-1127):
  * per query, K over [query, its m train neighbours]; alpha on the train
  * neighbours only; mean = K_xn K^-1 y_N, std = sqrt(K_00 - K_xn K^-1 K_nx).

@@ -88,6 +88,9 @@ SIGNATURES = [
     ("nngp_vecchia_workspace_bytes", _SZ, [_I64, _I, _I, _I, _I, _I, _I]),
     ("nngp_vecchia_loglik", _I, [_P, _P, _I64, _I, _I, _P, _I, _I64, _I64,
                                  ctypes.POINTER(NngpKernel), _D, _I, _I, _P, _P, _P, _P, _SZ, _P]),
+    ("nngp_vecchia_loglik_noalloc", _I, [_P, _P, _I64, _I, _I, _P, _I, _I64, _I64,
+                                         ctypes.POINTER(NngpKernel), _D, _I, _I, _P, _P, _P,
+                                         _P]),
     ("nngp_predict_workspace_bytes", _SZ, [_I64, _I, _I, _I, _I]),
     ("nngp_predict", _I, [_P, _P, _I64, _I, _I, _P, _I64, _P, _I,
                           ctypes.POINTER(NngpKernel), _D, _I, _P, _P, _P, _SZ, _P]),

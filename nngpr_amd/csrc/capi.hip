@@ -301,22 +301,44 @@ size_t nngp_vecchia_workspace_bytes(int64_t n_rows, int m, int d, int ny, int n_
     return c.used + 256;
 }
 
-int nngp_vecchia_loglik(const double* x, const double* y, int64_t n, int d, int ny,
-                        const int32_t* nn, int m, int64_t row0, int64_t row1,
-                        const nngp_kernel* kernel, double alpha, int precision,
-                        int eval_gradient, double* out_loglik, double* out_grad,
-                        double* out_point_loglik, void* workspace, size_t workspace_bytes,
-                        void* stream) {
-    hipStream_t st = (hipStream_t)stream;
+}  // extern "C"
+
+namespace {
+
+// Argument checks of the log-likelihood entry points.
+int check_loglik_args(int64_t n, int d, int ny, int m, int64_t row0, int64_t row1,
+                      const nngp_kernel* kernel, int precision, int eval_gradient,
+                      const double* out_loglik, const double* out_grad, KernParams* kp) {
     if (d < 1 || d > NNGP_MAX_DIM || ny < 1 || m < 1 || m > NNGP_MAX_NN || row0 < 0 ||
         row1 < row0 || row1 > n || !out_loglik)
         return NNGP_ERR_INVALID_ARGUMENT;
     if (precision != NNGP_PRECISION_FP64 && precision != NNGP_PRECISION_FP32_REFINED)
         return NNGP_ERR_INVALID_ARGUMENT;
-    KernParams kp;
-    int rc = to_kern_params(kernel, d, &kp);
+    const int rc = to_kern_params(kernel, d, kp);
     if (rc) return rc;
-    if (eval_gradient && kp.n_theta > 0 && !out_grad) return NNGP_ERR_INVALID_ARGUMENT;
+    if (eval_gradient && kp->n_theta > 0 && !out_grad) return NNGP_ERR_INVALID_ARGUMENT;
+    return NNGP_OK;
+}
+
+// out_loglik and out_grad[0, n_theta) become NaN when the call met a block that
+// could not be factored or a neighbour index outside [-1, n).
+__global__ void mark_failed_kernel(const WsHeader* h, double* out_ll, double* out_grad,
+                                   int n_theta) {
+    const bool failed = h->not_pd > 0 || h->bad_index > 0;
+    if (!failed) return;
+    const int i = threadIdx.x;
+    if (i == 0) *out_ll = __builtin_nan("");
+    if (out_grad)
+        for (int q = i; q < n_theta; q += blockDim.x) out_grad[q] = __builtin_nan("");
+}
+
+// The fused log-likelihood pass of rows [row0, row1) over a checked argument set,
+// with its scratch carved from (workspace, workspace_bytes).
+int run_loglik(const double* x, const double* y, int64_t n, int d, int ny, const int32_t* nn,
+               int m, int64_t row0, int64_t row1, const KernParams& kp, double alpha,
+               int precision, int eval_gradient, double* out_loglik, double* out_grad,
+               double* out_point_loglik, void* workspace, size_t workspace_bytes,
+               bool mark_failed, hipStream_t st) {
     const int64_t npts = row1 - row0;
     Carver c(workspace, workspace_bytes);
     VecLayout L = carve_vec(c, npts, m, d, kp.n_theta, true, precision);
@@ -341,15 +363,90 @@ int nngp_vecchia_loglik(const double* x, const double* y, int64_t n, int d, int 
     a.pt = L.pt;
     a.pt_stride = 1 + kp.n_theta;
     a.hdr = L.hdr;
-    rc = run_fused(a, L, precision, st);
+    const int rc = run_fused(a, L, precision, st);
     if (rc) return rc;
     launch_reduce(L.pt, npts, 1 + kp.n_theta, L.partial, out_loglik,
                   eval_gradient ? out_grad : nullptr, st);
     if (out_point_loglik && npts > 0)
         hipLaunchKernelGGL(extract_col_kernel, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0,
                            st, L.pt, 1 + kp.n_theta, npts, out_point_loglik);
+    if (mark_failed)
+        hipLaunchKernelGGL(mark_failed_kernel, dim3(1), dim3(64), 0, st, L.hdr, out_loglik,
+                           eval_gradient ? out_grad : nullptr, kp.n_theta);
     NNGP_HIP_CHECK(hipGetLastError());
     return NNGP_OK;
+}
+
+// Scratch of nngp_vecchia_loglik_noalloc: one copy per GPU, held until the
+// process ends.  Host code never passes `scratch` itself to a kernel (on the host
+// it names a shadow variable): it uses the device address that
+// hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) unsigned char scratch[NNGP_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Device address of the current device's scratch, looked up once per device.
+int scratch_address(void** out) {
+    int dev = 0;
+    NNGP_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxScratchDevices) return NNGP_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        NNGP_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return NNGP_ERR_HIP;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return NNGP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nngp_vecchia_loglik(const double* x, const double* y, int64_t n, int d, int ny,
+                        const int32_t* nn, int m, int64_t row0, int64_t row1,
+                        const nngp_kernel* kernel, double alpha, int precision,
+                        int eval_gradient, double* out_loglik, double* out_grad,
+                        double* out_point_loglik, void* workspace, size_t workspace_bytes,
+                        void* stream) {
+    KernParams kp;
+    const int rc = check_loglik_args(n, d, ny, m, row0, row1, kernel, precision, eval_gradient,
+                                     out_loglik, out_grad, &kp);
+    if (rc) return rc;
+    return run_loglik(x, y, n, d, ny, nn, m, row0, row1, kp, alpha, precision, eval_gradient,
+                      out_loglik, out_grad, out_point_loglik, workspace, workspace_bytes, false,
+                      (hipStream_t)stream);
+}
+
+int nngp_vecchia_loglik_noalloc(const double* x, const double* y, int64_t n, int d, int ny,
+                                const int32_t* nn, int m, int64_t row0, int64_t row1,
+                                const nngp_kernel* kernel, double alpha, int precision,
+                                int eval_gradient, double* out_loglik, double* out_grad,
+                                double* out_point_loglik, void* stream) {
+    KernParams kp;
+    int rc = check_loglik_args(n, d, ny, m, row0, row1, kernel, precision, eval_gradient,
+                               out_loglik, out_grad, &kp);
+    if (rc) return rc;
+    Carver c(nullptr, 0);
+    carve_vec(c, row1 - row0, m, d, kp.n_theta, true, precision);
+    const size_t need = c.used + 256;
+    if (need > NNGP_NOALLOC_SCRATCH_BYTES) {
+        std::snprintf(g_last_error, sizeof(g_last_error),
+                      "nngp_vecchia_loglik_noalloc: %zu rows x %d neighbours need %zu bytes of "
+                      "scratch; the library holds %zu per GPU",
+                      (size_t)(row1 - row0), m, need, (size_t)NNGP_NOALLOC_SCRATCH_BYTES);
+        return NNGP_ERR_WORKSPACE_TOO_SMALL;
+    }
+    void* ptr = nullptr;
+    rc = scratch_address(&ptr);
+    if (rc) return rc;
+    return run_loglik(x, y, n, d, ny, nn, m, row0, row1, kp, alpha, precision, eval_gradient,
+                      out_loglik, out_grad, out_point_loglik, ptr, NNGP_NOALLOC_SCRATCH_BYTES, true,
+                      (hipStream_t)stream);
 }
 
 size_t nngp_predict_workspace_bytes(int64_t n_query, int m, int d, int ny, int precision) {
