@@ -89,6 +89,12 @@ class TorchModule:
     def full(self, shape, value, dtype=np.float64):
         return torch.full(tuple(shape), value, dtype=self._dt(dtype), device=self.torch_device)
 
+    def ones_like(self, x):
+        return torch.ones_like(x)
+
+    def zeros_like(self, x):
+        return torch.zeros_like(x)
+
     def eye(self, n, dtype=np.float64):
         return torch.eye(n, dtype=self._dt(dtype), device=self.torch_device)
 

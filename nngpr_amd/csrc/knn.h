@@ -125,9 +125,10 @@ size_t knn_refs_bytes(int64_t n_ref, int d, int precision);
 int knn_prepare_refs(const double* ref, int64_t n_ref, int d, const double* scale, int n_scale,
                      int precision, void* mem, size_t bytes, KnnRefs* out, hipStream_t st);
 // with_cut: budget for searches with a Vecchia cut (nngp_knn); the serving
-// model's query searches have none.
+// model's query searches have none.  grid_refs: the refs use the uniform grid
+// (KnnRefs::use_grid), so no scan buffer is budgeted when k fits the grid.
 size_t knn_search_bytes(int64_t n_ref, int64_t n_query, int d, int k, int precision, bool same,
-                        bool with_cut);
+                        bool with_cut, bool grid_refs);
 // query == NULL searches the ref set itself (Vecchia self-search).
 int knn_search(const KnnRefs& R, const double* query, int64_t n_query, int k, int has_cut,
                int64_t ref_cut, int32_t* out_idx, void* ws, size_t ws_bytes, WsHeader* hdr,

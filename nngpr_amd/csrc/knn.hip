@@ -1014,9 +1014,16 @@ struct QueryMem {
     GridSearchMem grid;
 };
 
+// Query-side buffers of a search.  grid_refs: the refs use the uniform grid and
+// k fits it, so the scans never run and none of their buffers is carved.
 QueryMem carve_query(Carver& c, int64_t n_ref, int64_t n_query, int d, int k, int precision,
-                     bool same, bool with_cut) {
+                     bool same, bool with_cut, bool grid_refs) {
     QueryMem Q{};
+    if (grid_refs && grid_supported(d, k)) {
+        if (!same) Q.u64 = c.take<double>((size_t)n_query * d + 1);
+        carve_grid_search(c, n_ref, n_query, k, with_cut, &Q.grid);
+        return Q;
+    }
     const int64_t qpad = round_up(std::max<int64_t>(n_query, 1), 64) + 64;
     const int dp = scan_dim(d);
     const bool fast = use_fast_scan(d, k);
@@ -1049,9 +1056,12 @@ QueryMem carve_query(Carver& c, int64_t n_ref, int64_t n_query, int d, int k, in
     }
     Q.lkey = c.take<double>((size_t)k * kFallbackThreads);
     Q.lidx = c.take<int32_t>((size_t)k * kFallbackThreads);
-    if (grid_supported(d, k)) carve_grid_search(c, n_ref, n_query, k, with_cut, &Q.grid);
     return Q;
 }
+
+// Whether refs of d features prepared now would use the grid (the search method
+// is read when refs are prepared).
+bool refs_would_use_grid(int d) { return d <= 3 && knn_method() == NNGP_KNN_AUTO; }
 
 std::atomic<int> g_knn_method{NNGP_KNN_AUTO};
 
@@ -1087,7 +1097,7 @@ int knn_prepare_refs(const double* ref, int64_t n_ref, int d, const double* scal
     Carver c(mem, bytes);
     RefMem M = carve_refs(c, n_ref, d, precision);
     if (!c.ok() || !mem) return NNGP_ERR_WORKSPACE_TOO_SMALL;
-    const bool grid = d <= 3 && knn_method() == NNGP_KNN_AUTO;
+    const bool grid = refs_would_use_grid(d);
     if (grid) {  // the grid search needs the scaled coordinates only
         M.s64 = nullptr;
         M.s32 = nullptr;
@@ -1132,9 +1142,9 @@ int knn_prepare_refs(const double* ref, int64_t n_ref, int d, const double* scal
 }
 
 size_t knn_search_bytes(int64_t n_ref, int64_t n_query, int d, int k, int precision, bool same,
-                        bool with_cut) {
+                        bool with_cut, bool grid_refs) {
     Carver c(nullptr, 0);
-    carve_query(c, n_ref, n_query, d, k, precision, same, with_cut);
+    carve_query(c, n_ref, n_query, d, k, precision, same, with_cut, grid_refs);
     return c.used + 256;
 }
 
@@ -1144,8 +1154,9 @@ int knn_search(const KnnRefs& R, const double* query, int64_t n_query, int k, in
     const int d = R.d;
     if (k < 1 || k > NNGP_MAX_K || n_query < 0) return NNGP_ERR_INVALID_ARGUMENT;
     const bool same = query == nullptr;
+    const bool grid_path = R.use_grid && grid_supported(d, k);
     Carver c(ws, ws_bytes);
-    QueryMem Q = carve_query(c, R.n, n_query, d, k, R.precision, same, has_cut != 0);
+    QueryMem Q = carve_query(c, R.n, n_query, d, k, R.precision, same, has_cut != 0, grid_path);
     if (!c.ok() || (!ws && c.used > 0)) return NNGP_ERR_WORKSPACE_TOO_SMALL;
     if (n_query == 0) return NNGP_OK;
     if (R.n == 0) {
@@ -1155,7 +1166,8 @@ int knn_search(const KnnRefs& R, const double* query, int64_t n_query, int k, in
         NNGP_HIP_CHECK(hipGetLastError());
         return NNGP_OK;
     }
-    const bool fast = use_fast_scan(d, k) && (R.s32 || R.s64);
+    // the grid path reads only the query's scaled coordinates (no scan layouts)
+    const bool fast = !grid_path && use_fast_scan(d, k) && (R.s32 || R.s64);
     const bool screen = fast && R.precision == NNGP_PRECISION_FP32_REFINED;
     const int dp = scan_dim(d);
     const int64_t qpad = round_up(n_query, 64) + 64;
@@ -1187,7 +1199,7 @@ int knn_search(const KnnRefs& R, const double* query, int64_t n_query, int k, in
     }
     NNGP_HIP_CHECK(hipGetLastError());
 
-    if (R.use_grid && grid_supported(d, k)) {
+    if (grid_path) {
         int64_t row0 = 0;
         if (has_cut) {
             // Rows whose limit q + cut is below the first grid band: brute-force
@@ -1338,8 +1350,9 @@ size_t knn_workspace_bytes(int64_t n_ref, int64_t n_query, int d, int k, int pre
                            bool same) {
     Carver c(nullptr, 0);
     c.take<WsHeader>(1);
+    const bool grid = refs_would_use_grid(d);
     return c.used + knn_refs_bytes(n_ref, d, precision) +
-           knn_search_bytes(n_ref, n_query, d, k, precision, same, true) + 512;
+           knn_search_bytes(n_ref, n_query, d, k, precision, same, true, grid) + 512;
 }
 
 int knn_run(const double* ref, int64_t n_ref, const double* query, int64_t n_query, int d,
@@ -1354,7 +1367,10 @@ int knn_run(const double* ref, int64_t n_ref, const double* query, int64_t n_que
     WsHeader* hdr = c.take<WsHeader>(1);
     const size_t rb = knn_refs_bytes(n_ref, d, precision);
     char* rmem = c.take<char>(rb);
-    const size_t qb = knn_search_bytes(n_ref, n_query, d, k, precision, same, has_cut != 0);
+    // sized for the method read now; knn_search carves for the refs' actual
+    // method and reports a workspace too small if the method changed in between
+    const size_t qb = knn_search_bytes(n_ref, n_query, d, k, precision, same, has_cut != 0,
+                                       refs_would_use_grid(d));
     char* qmem = c.take<char>(qb);
     if (!c.ok()) return NNGP_ERR_WORKSPACE_TOO_SMALL;
     if (reset_stats) hipLaunchKernelGGL(zero_header_kernel, dim3(1), dim3(64), 0, st, hdr);

@@ -264,16 +264,19 @@ def test_grid_with_outliers(case, cut):
 
 @pytest.mark.parametrize("prec", [0, 1])
 @pytest.mark.parametrize("cut", [None, 0, 2, -1])
-def test_ref_stripes_small_inputs(prec, cut):
+@pytest.mark.parametrize("k", [1, 7, 32, 64])
+def test_ref_stripes_small_inputs(prec, cut, k):
     """Few query blocks (n = 9,000, d = 18): the scans split the refs into stripes and
     merge the stripes' lists; the indices equal the oracle's, including the first
     rows of the Vecchia cut (whose limits end inside the first stripe) and
-    duplicated points across stripes (ties go to the lower index)."""
+    duplicated points across stripes (ties go to the lower index).  k = 1 and 7
+    take the smallest lists (16 / 24 screen candidates), k = 64 the widest (72
+    screen candidates, 64 exact)."""
     rng = np.random.default_rng(41 + prec)
     n = 9000
     x = rng.normal(size=(n, 18))
     x[7000:7010] = x[10:20]        # exact duplicates in another stripe
     ls = rng.uniform(0.5, 2.0, 18)
     with hip_ops.knn_method(hip_ops.KNN_BRUTE_FORCE):
-        got = hip_knn_self(x, ls, 32, cut, prec)
-    np.testing.assert_array_equal(got, vo.knn(x, x, ls, 32, ref_cut=cut))
+        got = hip_knn_self(x, ls, k, cut, prec)
+    np.testing.assert_array_equal(got, vo.knn(x, x, ls, k, ref_cut=cut))
