@@ -159,6 +159,57 @@ def fit_predict_restarts(ds, cases=((200_000, 32), (434_874, 32), (200_000, 64))
     return out
 
 
+def fit_loop(ds, maxiter=60):
+    """L-BFGS-B fit of White + C * RBF (isotropic, 3 theta, all free) on the
+    road-mixture training set, from the workload's kernel: objective evaluations
+    per second through the estimator (validation, kernel decomposition, kNN, fused
+    pass, host transfers), with one kNN search reused across the evaluations
+    (exact rescale of k + 8 candidates) and with a search at every evaluation as
+    the reference does (base_nngpr.py:530-548).  Both fits must follow the same
+    path: theta and the log-likelihood are compared bit for bit."""
+    import numpy as np
+    import scipy.optimize
+    from nngpr_amd import TorchNngpr
+    out = {"workload": "road_mixture n=%d, White(0.01)+C(1.0)*RBF(0.1), m=32, L-BFGS-B "
+                       "maxiter=%d, reference default precision" % (ds["x_train"].shape[0], maxiter)}
+    fits = {}
+    for reuse in (True, False):
+        counter = {"evals": 0, "seconds": 0.0}
+
+        def opt(obj, theta0, bounds, counter=counter):
+            def f(th):
+                counter["evals"] += 1
+                t = time.perf_counter()
+                r = obj(th)
+                counter["seconds"] += time.perf_counter() - t
+                return r
+            r = scipy.optimize.minimize(f, theta0, method="L-BFGS-B", jac=True, bounds=bounds,
+                                        options={"maxiter": maxiter})
+            return r.x, r.fun
+
+        est = TorchNngpr("cuda:0", kernel=ds["kernel"], num_nn=32, alpha=ds["alpha"],
+                         optimizer=opt)
+        est._knn_reuse = reuse
+        est.fit(ds["x_train"][:2000], ds["y_train"][:2000])  # warm-up (compiles, allocator)
+        counter["evals"], counter["seconds"] = 0, 0.0
+        t0 = time.perf_counter()
+        est.fit(ds["x_train"], ds["y_train"])
+        t = time.perf_counter() - t0
+        fits[reuse] = est
+        key = "knn_reused" if reuse else "knn_per_eval"
+        # evals_per_s: objective calls over the time spent inside them (the first
+        # includes the one search); fit_seconds also holds validation, the upload
+        # and the optimizer's own work
+        out[key] = {"fit_seconds": t, "objective_evals": counter["evals"],
+                    "objective_seconds": counter["seconds"],
+                    "evals_per_s": counter["evals"] / counter["seconds"],
+                    "fit_loglik": float(est.log_marginal_likelihood_value_)}
+    out["identical_path"] = bool(
+        np.array_equal(fits[True].kernel_.theta, fits[False].kernel_.theta) and
+        fits[True].log_marginal_likelihood_value_ == fits[False].log_marginal_likelihood_value_)
+    return out
+
+
 def c4_eval(num_nn):
     """uniform_8d (n=2,000,000, d=8): one loglik+grad evaluation in the FP32 +
     FP64-refinement form, kNN (brute-force scans, d > 3) and fused pass timed
@@ -359,7 +410,7 @@ def main():
     cpu = None
     cpu_seq = None
     ALL_EXTRAS = {"modes", "serving", "fit_d18", "uniform_8d", "matern", "sequential",
-                  "fit_predict_restarts"}
+                  "fit_predict_restarts", "fit_loop"}
     want = set() if args.skip_extras else (
         ALL_EXTRAS if args.extras is None else set(args.extras.split(",")))
     if want - ALL_EXTRAS:
@@ -525,6 +576,8 @@ def main():
     serving = None
     if "serving" in want:
         serving = serve_sweep(ds, (50, 100, 200, 500, 1000, 2000))
+    if "fit_loop" in want:
+        extras["fit_loop_road_mixture"] = fit_loop(ds)
     if "fit_d18" in want:
         extras["correlated_gauss_d18"] = c3_fit()
     if "fit_predict_restarts" in want:

@@ -115,6 +115,33 @@ int nngp_knn(const double* ref, int64_t n_ref, const double* query, int64_t n_qu
              void* stream);
 
 /*
+ * Exact kNN lists at a new length scale from exact lists at an old one, for
+ * optimizer loops.  The reference searches again at every log-likelihood
+ * evaluation of fit (base_nngpr.py:530-548, RbfAccelerator.find_nearest_neighbors
+ * with the new length scales, cuda_kernel_accelerators.py:196-381).
+ *   cand (n_query x K int32, device): the output of nngp_knn with k = K
+ *     (k <= K <= 72) at scale_old for the same ref, query and cut.
+ *   state (device, nngp_knn_rescale_state_bytes): nngp_knn_rescale_init fills it
+ *     once per cand (per row: the K-th candidate key and whether the first k + 1
+ *     keys are separated beyond rounding); nngp_knn_rescale reads it and also
+ *     uses it as scratch (calls sharing a state must not overlap).
+ *   nngp_knn_rescale writes to out_idx (n_query x k) exactly the lists
+ *     nngp_knn(k, scale_new) returns: under an isotropic change rows marked
+ *     stable keep their lists; other rows re-rank their K candidates at the new
+ *     scale and are certified by a lower bound on every other ref's key; rows
+ *     that fail are recomputed exactly (their count: nngp_read_stats(state)).
+ * scale_old / scale_new: host arrays of n_scale (0, 1 or d) positive values.
+ */
+size_t nngp_knn_rescale_state_bytes(int64_t n_ref, int64_t n_query, int d, int k, int K);
+int nngp_knn_rescale_init(const double* ref, int64_t n_ref, const double* query, int64_t n_query,
+                          int d, const double* scale, int n_scale, const int32_t* cand, int K,
+                          int k, void* state, size_t state_bytes, void* stream);
+int nngp_knn_rescale(const double* ref, int64_t n_ref, const double* query, int64_t n_query,
+                     int d, const double* scale_old, const double* scale_new, int n_scale,
+                     int has_cut, int64_t ref_cut, const int32_t* cand, int K, int k,
+                     int32_t* out_idx, void* state, size_t state_bytes, void* stream);
+
+/*
  * Search method of nngp_knn and of the serving model's query search,
  * process-wide, read when the refs are prepared (at each nngp_knn call, at
  * nngp_model_create).  AUTO: a uniform-grid exact search for d <= 3 and k <= 64

@@ -84,6 +84,10 @@ SIGNATURES = [
     ("nngp_abi_version", _I, []),
     ("nngp_knn_workspace_bytes", _SZ, [_I64, _I64, _I, _I, _I]),
     ("nngp_knn", _I, [_P, _I64, _P, _I64, _I, _P, _I, _I, _I, _I64, _I, _P, _P, _SZ, _P]),
+    ("nngp_knn_rescale_state_bytes", _SZ, [_I64, _I64, _I, _I, _I]),
+    ("nngp_knn_rescale_init", _I, [_P, _I64, _P, _I64, _I, _P, _I, _P, _I, _I, _P, _SZ, _P]),
+    ("nngp_knn_rescale", _I, [_P, _I64, _P, _I64, _I, _P, _P, _I, _I, _I64, _P, _I, _I, _P, _P,
+                              _SZ, _P]),
     ("nngp_fill_nn_kernel", _I, [_P, _I64, _I, _P, _I64, _I, ctypes.POINTER(NngpKernel), _P, _P, _P]),
     ("nngp_vecchia_workspace_bytes", _SZ, [_I64, _I, _I, _I, _I, _I, _I]),
     ("nngp_vecchia_loglik", _I, [_P, _P, _I64, _I, _I, _P, _I, _I64, _I64,

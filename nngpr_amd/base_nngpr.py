@@ -270,18 +270,53 @@ class Nngpr(GaussianProcessRegressor):
             self._dev_cache = (key, x, y, {})
         return self._dev_cache[1], self._dev_cache[2], self._dev_cache[3]
 
+    # extra candidates per row kept for reusing a search at other length scales
+    _KNN_REUSE_MARGIN = 8
+    # a base whose last rescale recomputed more than this fraction of the rows is
+    # replaced by a new search at the next new scale
+    _KNN_REBASE_FRACTION = 0.02
+
     def _train_knn(self, kernel, x_dev, knn_cache, precision):
         """Vecchia conditioning sets: each row's nearest earlier rows
-        (base_nngpr.py:530-546, ref = x[:i1], query = x[i0:i1], ref_cut = i0)."""
+        (base_nngpr.py:530-546, ref = x[:i1], query = x[i0:i1], ref_cut = i0).
+
+        The reference searches again at every evaluation.  Here, in kernel space,
+        one search keeps k + 8 exact candidates per row, and the lists at later
+        length scales come from hip_ops.KnnRescaler: the same lists a new search
+        would return (certified per row, uncertified rows searched exactly), for
+        the cost of a copy under an isotropic change.  A base whose rescales
+        recompute many rows is replaced by a search at the next new scale."""
         n = x_dev.shape[0]
         k = max(1, min(self.num_nn, n - 1))
         scale = self._worker.knn_scale(kernel, self.nn_type)
-        key = (None if scale is None else tuple(scale.tolist()), k, self.nn_type, precision)
-        nn = knn_cache.get(key)
-        if nn is None:
-            knn_cache.clear()
+        key = (None if scale is None else tuple(scale.tolist()), k, self.nn_type)
+        lists = knn_cache.setdefault("lists", {})
+        nn = lists.get(key)
+        if nn is not None:
+            return nn
+        lists.clear()
+        K = k + self._KNN_REUSE_MARGIN
+        if scale is None or not getattr(self, "_knn_reuse", True) or \
+                K > hip_ops.RESCALE_MAX_K or k > nat.NNGP_MAX_K:
             nn = _knn_lists(x_dev, x_dev, scale, k, ref_cut=0, precision=precision)
-            knn_cache[key] = nn
+        else:
+            base = knn_cache.get("rescaler")
+            if base is not None and (base.k != k or base.scale.size != scale.size):
+                base = None
+            # the previous rescale's recomputed rows (read now: the stream has
+            # synchronised since, for the log-likelihood value)
+            if base is not None and knn_cache.pop("check", False) and \
+                    base.rows_recomputed() > self._KNN_REBASE_FRACTION * n:
+                base = None
+            if base is None:
+                base = hip_ops.KnnRescaler(x_dev, x_dev, scale, k, min(K, max(n - 1, k)),
+                                           ref_cut=0, precision=precision)
+                knn_cache["rescaler"] = base
+                nn = base.base_lists()
+            else:
+                nn = base.lists(scale)
+                knn_cache["check"] = True
+        lists[key] = nn
         return nn
 
     # ---------------------------------------------------------------------- fit
@@ -378,11 +413,14 @@ class Nngpr(GaussianProcessRegressor):
         ks, _ = self._worker.kernel_struct(kernel)
         nn = self._train_knn(kernel, x_dev, cache, knn_prec)
         ll, grad, _ = hip_ops.vecchia_loglik(x_dev, y_dev, nn, ks, float(self.alpha),
-                                             precision=fac_prec, eval_gradient=eval_gradient)
+                                             precision=fac_prec, eval_gradient=eval_gradient,
+                                             check_pd=False)
+        # one transfer for the value and the gradient, then the call's status
+        vals = (torch.cat([ll, grad]) if eval_gradient else ll).cpu().numpy()
+        hip_ops.check_status("vecchia", "log_marginal_likelihood", x_dev.device)
         if eval_gradient:
-            g = grad.cpu().numpy().astype(np.float64)
-            return float(ll.item()), g
-        return float(ll.item())
+            return float(vals[0]), vals[1:].astype(np.float64)
+        return float(vals[0])
 
     # ------------------------------------------------------------------ predict
     def predict(self, X, return_std=False, return_cov=False, conditioning_method=None):

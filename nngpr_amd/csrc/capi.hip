@@ -275,6 +275,29 @@ int nngp_knn(const double* ref, int64_t n_ref, const double* query, int64_t n_qu
                    out_idx, workspace, workspace_bytes, (hipStream_t)stream, true);
 }
 
+size_t nngp_knn_rescale_state_bytes(int64_t n_ref, int64_t n_query, int d, int k, int K) {
+    if (n_ref < 0 || n_query < 0 || d < 1 || d > NNGP_MAX_DIM || k < 1) return 0;
+    return rescale_state_bytes(n_ref, n_query, d, k, K);
+}
+
+int nngp_knn_rescale_init(const double* ref, int64_t n_ref, const double* query, int64_t n_query,
+                          int d, const double* scale, int n_scale, const int32_t* cand, int K,
+                          int k, void* state, size_t state_bytes, void* stream) {
+    if ((n_ref > 0 && !ref) || (n_query > 0 && (!query || !cand))) return NNGP_ERR_INVALID_ARGUMENT;
+    return rescale_init(ref, n_ref, query, n_query, d, scale, n_scale, cand, K, k, state,
+                        state_bytes, (hipStream_t)stream);
+}
+
+int nngp_knn_rescale(const double* ref, int64_t n_ref, const double* query, int64_t n_query,
+                     int d, const double* scale_old, const double* scale_new, int n_scale,
+                     int has_cut, int64_t ref_cut, const int32_t* cand, int K, int k,
+                     int32_t* out_idx, void* state, size_t state_bytes, void* stream) {
+    if ((n_ref > 0 && !ref) || (n_query > 0 && (!query || !cand || !out_idx)))
+        return NNGP_ERR_INVALID_ARGUMENT;
+    return rescale_run(ref, n_ref, query, n_query, d, scale_old, scale_new, n_scale, has_cut,
+                       ref_cut, cand, K, k, out_idx, state, state_bytes, (hipStream_t)stream);
+}
+
 int nngp_knn_set_method(int method) { return knn_set_method(method); }
 
 int nngp_knn_get_method(void) { return knn_method(); }

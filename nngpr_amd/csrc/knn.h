@@ -136,6 +136,39 @@ int knn_search(const KnnRefs& R, const double* query, int64_t n_query, int k, in
 
 size_t knn_workspace_bytes(int64_t n_ref, int64_t n_query, int d, int k, int precision,
                            bool same);
+
+// Exact recomputation of listed query rows: rows[0 .. *count), each with an upper
+// bound pair (bound_key, bound_idx) of its true k-th (key, index) pair (a pair of
+// an actual ref).  One wave per row keeps every pair at or below the bound and
+// ranks them; rows keeping more than 1,024 pairs go to the serial per-row kernel
+// (over_rows, scratch lkey / lidx of exact_rows_scratch_count(k) entries).
+struct ExactRows {
+    const int64_t* rows;
+    const unsigned long long* count;
+    const double* bound_key;
+    const int32_t* bound_idx;
+    int64_t* over_rows;              // n_query entries
+    unsigned long long* over_count;
+    double* lkey;
+    int32_t* lidx;
+};
+size_t exact_rows_scratch_count(int k);
+void launch_exact_rows(const double* ru64, const double* qu64, int d, int64_t n_ref,
+                       int64_t n_query, int has_cut, int64_t cut, int k, const ExactRows& er,
+                       int32_t* out_idx, hipStream_t st);
+
+// ---------------------------------------------------------------------------
+// Exact lists at a new scale from exact lists at an old one (knn_rescale.hip)
+// ---------------------------------------------------------------------------
+int rescale_max_candidates();
+size_t rescale_state_bytes(int64_t n_ref, int64_t n_query, int d, int k, int K);
+int rescale_init(const double* ref, int64_t n_ref, const double* query, int64_t n_query, int d,
+                 const double* scale, int n_scale, const int32_t* cand, int K, int k, void* state,
+                 size_t state_bytes, hipStream_t st);
+int rescale_run(const double* ref, int64_t n_ref, const double* query, int64_t n_query, int d,
+                const double* scale_old, const double* scale_new, int n_scale, int has_cut,
+                int64_t ref_cut, const int32_t* cand, int K, int k, int32_t* out_idx, void* state,
+                size_t state_bytes, hipStream_t st);
 int knn_run(const double* ref, int64_t n_ref, const double* query, int64_t n_query, int d,
             const double* scale, int n_scale, int k, int has_cut, int64_t ref_cut, int precision,
             int32_t* out_idx, void* ws, size_t ws_bytes, hipStream_t st, bool reset_stats);

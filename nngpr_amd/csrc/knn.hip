@@ -1076,6 +1076,45 @@ int check_common(int64_t n_ref, int d, int n_scale, int precision) {
 
 }  // namespace
 
+size_t exact_rows_scratch_count(int k) { return (size_t)k * kFallbackThreads; }
+
+void launch_exact_rows(const double* ru64, const double* qu64, int d, int64_t n_ref,
+                       int64_t n_query, int has_cut, int64_t cut, int k, const ExactRows& er,
+                       int32_t* out_idx, hipStream_t st) {
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, er.over_count, 1);
+    WaveFbArgs wf{};
+    wf.ru64 = ru64;
+    wf.qu64 = qu64;
+    wf.d = d;
+    wf.n_ref = n_ref;
+    wf.has_cut = has_cut;
+    wf.cut = cut;
+    wf.k = k;
+    wf.rows = er.rows;
+    wf.row_count = er.count;
+    wf.bound_key = er.bound_key;
+    wf.bound_idx = er.bound_idx;
+    wf.out_idx = out_idx;
+    wf.over_rows = er.over_rows;
+    wf.over_count = er.over_count;
+    hipLaunchKernelGGL(wave_fallback_kernel, dim3(2048), dim3(64 * kFbWaves), 0, st, wf);
+    SimpleArgs sa{};
+    sa.ru64 = ru64;
+    sa.qu64 = qu64;
+    sa.d = d;
+    sa.n_ref = n_ref;
+    sa.n_query = n_query;
+    sa.has_cut = has_cut;
+    sa.cut = cut;
+    sa.k = k;
+    sa.rows = er.over_rows;
+    sa.row_count = er.over_count;
+    sa.lkey = er.lkey;
+    sa.lidx = er.lidx;
+    sa.out_idx = out_idx;
+    hipLaunchKernelGGL(simple_knn_kernel, dim3(kFallbackThreads / 256), dim3(256), 0, st, sa);
+}
+
 int knn_method() { return g_knn_method.load(); }
 
 int knn_set_method(int method) {
@@ -1281,7 +1320,6 @@ int knn_search(const KnnRefs& R, const double* query, int64_t n_query, int k, in
         a.out_pk = Q.spk;
     }
     hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, Q.flag_count, 1);
-    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, Q.over_count, 1);
     timing_begin(TK_KNN_SCAN, st);
     const int rcs = launch_scan<float, true>(a, d, K, st);
     timing_end(TK_KNN_SCAN, st);
@@ -1308,38 +1346,16 @@ int knn_search(const KnnRefs& R, const double* query, int64_t n_query, int k, in
     timing_end(TK_KNN_RERANK, st);
     if (rcr != 0) return NNGP_ERR_UNSUPPORTED;
 
-    WaveFbArgs wf{};
-    wf.ru64 = R.u64;
-    wf.qu64 = qu64;
-    wf.d = d;
-    wf.n_ref = R.n;
-    wf.has_cut = has_cut;
-    wf.cut = ref_cut;
-    wf.k = k;
-    wf.rows = Q.flag_rows;
-    wf.row_count = Q.flag_count;
-    wf.bound_key = Q.flag_key;
-    wf.bound_idx = Q.flag_idx;
-    wf.out_idx = out_idx;
-    wf.over_rows = Q.over_rows;
-    wf.over_count = Q.over_count;
-    hipLaunchKernelGGL(wave_fallback_kernel, dim3(2048), dim3(64 * kFbWaves), 0, st, wf);
-
-    SimpleArgs sa{};
-    sa.ru64 = R.u64;
-    sa.qu64 = qu64;
-    sa.d = d;
-    sa.n_ref = R.n;
-    sa.n_query = n_query;
-    sa.has_cut = has_cut;
-    sa.cut = ref_cut;
-    sa.k = k;
-    sa.rows = Q.over_rows;
-    sa.row_count = Q.over_count;
-    sa.lkey = Q.lkey;
-    sa.lidx = Q.lidx;
-    sa.out_idx = out_idx;
-    hipLaunchKernelGGL(simple_knn_kernel, dim3(kFallbackThreads / 256), dim3(256), 0, st, sa);
+    ExactRows er{};
+    er.rows = Q.flag_rows;
+    er.count = Q.flag_count;
+    er.bound_key = Q.flag_key;
+    er.bound_idx = Q.flag_idx;
+    er.over_rows = Q.over_rows;
+    er.over_count = Q.over_count;
+    er.lkey = Q.lkey;
+    er.lidx = Q.lidx;
+    launch_exact_rows(R.u64, qu64, d, R.n, n_query, has_cut, ref_cut, k, er, out_idx, st);
     if (hdr) hipLaunchKernelGGL(add_count_kernel, dim3(1), dim3(1), 0, st, Q.flag_count,
                                 &hdr->knn_rows_recomputed);
     NNGP_HIP_CHECK(hipGetLastError());

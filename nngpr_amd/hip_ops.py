@@ -104,6 +104,74 @@ def knn(ref, query, scale, k, ref_cut=None, precision=PRECISION_FP32_REFINED, ou
     return out
 
 
+RESCALE_MAX_K = 72
+
+
+class KnnRescaler:
+    """Exact neighbour lists at new length scales from one search's top-K lists
+    (nngp_knn_rescale_init / nngp_knn_rescale in include/nngp.h).
+
+    The search runs once with K = k + margin candidates at `scale`; `lists(s)`
+    then returns exactly what `knn(ref, query, s, k, ref_cut)` would, re-ranking
+    or copying the stored candidates and recomputing only the rows whose
+    certificate fails (`last_rows_recomputed`).
+    """
+
+    def __init__(self, ref, query, scale, k, K, ref_cut=None,
+                 precision=PRECISION_FP32_REFINED):
+        if not (k <= K <= RESCALE_MAX_K):
+            raise ValueError("need k <= K <= %d" % RESCALE_MAX_K)
+        self.ref = _f64(ref)
+        self.query = self.ref if query is ref else _f64(query)
+        self.dev = _require_device(self.ref, self.query)
+        self.d = self.ref.shape[1]
+        self.k, self.K = int(k), int(K)
+        self.ref_cut = ref_cut
+        self.scale = _scale_array(scale, self.d)
+        if self.scale is None:
+            raise ValueError("rescaling needs a length scale")
+        self.cand = knn(self.ref, self.query, self.scale, self.K, ref_cut=ref_cut,
+                        precision=precision)
+        L = nat.lib()
+        nq, nr = self.query.shape[0], self.ref.shape[0]
+        nbytes = L.nngp_knn_rescale_state_bytes(nr, nq, self.d, self.k, self.K)
+        self.state = torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=self.dev)
+        rc = L.nngp_knn_rescale_init(_p(self.ref), nr, _p(self.query), nq, self.d,
+                                     self.scale.ctypes.data_as(ctypes.c_void_p), self.scale.size,
+                                     _p(self.cand), self.K, self.k, _p(self.state),
+                                     self.state.numel(), _stream(self.dev))
+        nat.check(rc, "nngp_knn_rescale_init")
+        self.last_rows_recomputed = 0
+
+    def base_lists(self):
+        """The exact top-k lists at the search's own scale."""
+        return self.cand[:, :self.k].contiguous()
+
+    def lists(self, scale, out=None):
+        sc = _scale_array(scale, self.d)
+        if sc is None or sc.size != self.scale.size:
+            raise ValueError("scale must have as many entries as the base scale")
+        nq, nr = self.query.shape[0], self.ref.shape[0]
+        if out is None:
+            out = torch.empty((nq, self.k), dtype=torch.int32, device=self.dev)
+        cut = self.ref_cut
+        rc = nat.lib().nngp_knn_rescale(
+            _p(self.ref), nr, _p(self.query), nq, self.d,
+            self.scale.ctypes.data_as(ctypes.c_void_p), sc.ctypes.data_as(ctypes.c_void_p),
+            sc.size, 0 if cut is None else 1, 0 if cut is None else int(cut), _p(self.cand),
+            self.K, self.k, _p(out), _p(self.state), self.state.numel(), _stream(self.dev))
+        nat.check(rc, "nngp_knn_rescale")
+        return out
+
+    def rows_recomputed(self):
+        """Rows of the last `lists` call recomputed by the exact per-row search
+        (synchronises the stream)."""
+        st = nat.NngpStats()
+        nat.check(nat.lib().nngp_read_stats(_p(self.state), ctypes.byref(st), _stream(self.dev)),
+                  "nngp_read_stats")
+        return int(st.knn_rows_recomputed)
+
+
 def stats(ws_slot, dev=None):
     """Counters of the last call that used workspace slot `ws_slot` ("knn",
     "vecchia", "predict", ...): kNN rows recomputed exactly, points re-factored in
@@ -149,6 +217,17 @@ def _check_pd(ws, dev, what):
         raise ValueError("%s: a neighbour index lies outside [-1, number of rows)" % what)
     nat.check(rc, "nngp_read_stats")
     return st
+
+
+def check_status(ws_slot, what, dev=None):
+    """Raise for the last call on workspace slot `ws_slot` as check_pd=True would
+    (LinAlgError for a block that could not be factored, ValueError for an index
+    out of range); for callers that launch with check_pd=False and read their
+    results first."""
+    dev = dev or torch.device("cuda", torch.cuda.current_device())
+    buf = _workspaces.get((str(dev), ws_slot))
+    if buf is not None:
+        _check_pd(buf, dev, what)
 
 
 def vecchia_loglik(x, y, nn, kstruct, alpha, precision=PRECISION_FP64, eval_gradient=False,
