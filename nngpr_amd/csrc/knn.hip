@@ -795,22 +795,6 @@ __global__ void fill_i32_kernel(int32_t* p, int64_t n, int32_t v) {
     if (i < n) p[i] = v;
 }
 
-// Wave-wide bitonic stage over (key, index) pairs, one pair per lane: lanes l and
-// l ^ stride exchange; the lower lane of an ascending block keeps the smaller pair.
-__device__ __forceinline__ void wave_pair_step(double& key, int& idx, int lane, int size,
-                                               int stride) {
-    const double ok = __shfl_xor(key, stride, 64);
-    const int oi = __shfl_xor(idx, stride, 64);
-    const bool up = (lane & size) == 0;
-    const bool lower = (lane & stride) == 0;
-    const bool other_less = pair_less(ok, oi, key, idx);
-    const bool other_more = pair_less(key, idx, ok, oi);
-    if (lower == up ? other_less : other_more) {
-        key = ok;
-        idx = oi;
-    }
-}
-
 // Exact top-k (k <= 64) of query rows [qlo, qlo + gridDim.x) whose ref limit
 // (row + cut, at most a few hundred refs) is below the first grid band: one wave
 // per row, the refs in chunks of 64 lanes (exact_key), each chunk sorted by (key,

@@ -112,6 +112,22 @@ __device__ __forceinline__ void pair_merge(double (&keys)[K], int (&ids)[K], dou
     }
 }
 
+// Wave-wide bitonic stage over (key, index) pairs, one pair per lane: lanes l and
+// l ^ stride exchange; the lower lane of an ascending block keeps the smaller pair.
+__device__ __forceinline__ void wave_pair_step(double& key, int& idx, int lane, int size,
+                                               int stride) {
+    const double ok = __shfl_xor(key, stride, 64);
+    const int oi = __shfl_xor(idx, stride, 64);
+    const bool up = (lane & size) == 0;
+    const bool lower = (lane & stride) == 0;
+    const bool other_less = pair_less(ok, oi, key, idx);
+    const bool other_more = pair_less(key, idx, ok, oi);
+    if (lower == up ? other_less : other_more) {
+        key = ok;
+        idx = oi;
+    }
+}
+
 // FP64 key of a (ref, query) pair: fma(t_{d-1}, t_{d-1}, ... fma(t_1, t_1, t_0 * t_0)),
 // t_f = u_ref,f - u_query,f.
 __device__ __forceinline__ double exact_key(const double* ur, const double* uq, int d) {
