@@ -337,6 +337,12 @@ int nngp_read_stats(const void* workspace, nngp_stats* out, void* stream);
  */
 int nngp_timing_enable(int on);
 int nngp_timing_read(int kernel, int reset, double* total_ms, int64_t* launches);
+/*
+ * Work measured inside a timed kernel while timing is on, summed over its
+ * launches: for the kNN grid search (4) the (query, ref) pairs whose key was
+ * computed; 0 for the others.  Synchronises the device.
+ */
+int nngp_timing_work(int kernel, int reset, int64_t* units);
 
 #ifdef __cplusplus
 }

@@ -115,6 +115,7 @@ SIGNATURES = [
     ("nngp_knn_get_method", _I, []),
     ("nngp_timing_enable", _I, [_I]),
     ("nngp_timing_read", _I, [_I, _I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I64)]),
+    ("nngp_timing_work", _I, [_I, _I, ctypes.POINTER(_I64)]),
 ]
 
 TIMED_KNN_SCAN, TIMED_KNN_RERANK, TIMED_LOGLIK, TIMED_PREDICT = 0, 1, 2, 3
@@ -144,6 +145,14 @@ def timing_read(kernel, reset=True):
     check(lib().nngp_timing_read(kernel, 1 if reset else 0, ctypes.byref(ms), ctypes.byref(n)),
           "nngp_timing_read")
     return ms.value, n.value
+
+
+def timing_work(kernel, reset=True):
+    """Work units measured inside a timed kernel since the last reset (the grid
+    search: (query, ref) pairs keyed)."""
+    u = ctypes.c_int64()
+    check(lib().nngp_timing_work(kernel, 1 if reset else 0, ctypes.byref(u)), "nngp_timing_work")
+    return u.value
 
 _lib = None
 

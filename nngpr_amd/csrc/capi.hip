@@ -29,6 +29,7 @@ struct TimingState {
     bool on = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[TK_COUNT];
     hipEvent_t pending[TK_COUNT] = {};
+    unsigned long long* work = nullptr;  // device: one work counter per timed kernel
 };
 TimingState& timing() {
     static TimingState t;
@@ -45,6 +46,12 @@ void timing_begin(int id, hipStream_t st) {
     if (hipEventCreate(&e) != hipSuccess) return;
     (void)hipEventRecord(e, st);
     t.pending[id] = e;
+}
+
+unsigned long long* timing_work_counter(int id) {
+    TimingState& t = timing();
+    std::lock_guard<std::mutex> lk(t.mu);
+    return (t.on && t.work) ? t.work + id : nullptr;
 }
 
 void timing_end(int id, hipStream_t st) {
@@ -233,7 +240,33 @@ int nngp_abi_version(void) { return NNGP_ABI_VERSION; }
 int nngp_timing_enable(int on) {
     TimingState& t = timing();
     std::lock_guard<std::mutex> lk(t.mu);
+    if (on && !t.work) {
+        // measurement support only: the counters are allocated once, zeroed from
+        // the host
+        void* p = nullptr;
+        NNGP_HIP_CHECK(hipMalloc(&p, sizeof(unsigned long long) * TK_COUNT));
+        const unsigned long long zero[TK_COUNT] = {};
+        NNGP_HIP_CHECK(hipMemcpy(p, zero, sizeof(zero), hipMemcpyHostToDevice));
+        t.work = static_cast<unsigned long long*>(p);
+    }
     t.on = on != 0;
+    return NNGP_OK;
+}
+
+int nngp_timing_work(int kernel, int reset, int64_t* units) {
+    if (kernel < 0 || kernel >= TK_COUNT || !units) return NNGP_ERR_INVALID_ARGUMENT;
+    TimingState& t = timing();
+    std::lock_guard<std::mutex> lk(t.mu);
+    *units = 0;
+    if (!t.work) return NNGP_OK;
+    NNGP_HIP_CHECK(hipDeviceSynchronize());
+    unsigned long long v = 0;
+    NNGP_HIP_CHECK(hipMemcpy(&v, t.work + kernel, sizeof(v), hipMemcpyDeviceToHost));
+    *units = (int64_t)v;
+    if (reset) {
+        const unsigned long long zero = 0;
+        NNGP_HIP_CHECK(hipMemcpy(t.work + kernel, &zero, sizeof(zero), hipMemcpyHostToDevice));
+    }
     return NNGP_OK;
 }
 

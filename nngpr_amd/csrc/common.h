@@ -35,6 +35,9 @@ enum TimedKernel {
 };
 void timing_begin(int id, hipStream_t st);
 void timing_end(int id, hipStream_t st);
+// Device counter of measured work units of a timed kernel (the grid search: the
+// (query, ref) pairs it keyed), or NULL when timing is off.
+unsigned long long* timing_work_counter(int id);
 
 template <typename T>
 __device__ __forceinline__ T dev_inf();

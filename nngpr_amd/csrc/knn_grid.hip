@@ -495,6 +495,9 @@ struct SearchArgs {
     int64_t cut;
     int k;
     int32_t* out_idx;
+    // when non-null, every wave adds the number of (query, ref) pairs its lanes
+    // examined (the measured work of the search)
+    unsigned long long* pair_count;
 };
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
@@ -610,6 +613,7 @@ void lane_knn_kernel(SearchArgs a) {
     int ids[K];
     init_list<K>(keys, ids, pad);
 
+    uint32_t examined = 0;  // refs this lane keyed
     if (__builtin_amdgcn_ballot_w64(active)) {
         const double h = P.h, tol = P.tol;
         const int32_t* cs = bd0.cell_start;
@@ -833,6 +837,7 @@ void lane_knn_kernel(SearchArgs a) {
                         ++cnt;
                     }
                 }
+                examined += (uint32_t)min(CH, e - p);
                 p += CH;
             }
             // a lane whose run is finished finds its next run in the same step
@@ -900,6 +905,12 @@ void lane_knn_kernel(SearchArgs a) {
     int pad_out = K - a.k;
     asm volatile("" : "+v"(pad_out));
     if (jv) store_list<K>(ids, pad_out, a.out_idx + q * a.k);
+    if (a.pair_count) {
+        unsigned long long s = examined;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        if (lane == 0 && s) atomicAdd(a.pair_count, s);
+    }
 }
 
 // Estimated cost of each search wave: the largest number of refs in the 3 x 3 x 3
@@ -1180,6 +1191,7 @@ int grid_search(const GridRefMem& G, const double* ru64, int64_t n_ref, int d,
     if (n_r > 0) run_sorts(ru64, G.bbox, d, rjobs, n_r, true, st);
     run_sorts(qu64, G.bbox, d, qjobs, n_q, false, st);
     timing_end(TK_KNN_GRID_BUILD, st);
+    sa.pair_count = timing_work_counter(TK_KNN_GRID);
     timing_begin(TK_KNN_GRID, st);
     launch_search(sa, d, M, st);
     timing_end(TK_KNN_GRID, st);

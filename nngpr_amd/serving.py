@@ -17,8 +17,9 @@ Design
   number of slots and their capacity are derived from an HBM budget, so a small
   budget makes requests queue rather than fail.  Requests that arrive while other
   batches are on the device join the next free slot: continuous batching.
-* Each slot's batch is padded to a bucket size (powers of 4 from 256 up to the slot
-  capacity) and replayed from a HIP graph captured once per (slot, bucket): the
+* Each slot's batch is padded to a bucket size (256 * sqrt(2)^i rounded to a
+  multiple of 64, up to the slot capacity: at most ~1.41x padding) and replayed
+  from a HIP graph captured once per (slot, bucket): the
   kNN screen, re-rank, fallback and fused prediction kernels of the whole batch are
   one graph launch.  Graph capture can be disabled (`use_graphs=False`).
 * Every query's result is independent of the other queries in its batch, so a
@@ -164,22 +165,32 @@ class _Slot:
         self.parts = None  # list of (request, row0, rows, offset) of the batch in flight
 
     def bucket(self, n):
-        b = 256
-        while b < n:
-            b *= 4
-        return min(b, self.capacity)
+        """Smallest captured batch size that holds n points."""
+        for b in self.buckets():
+            if b >= n:
+                return b
+        return self.capacity
 
     def _enqueue_predict(self, b):
         self.model.predict_into(self.q_dev[:b], self.mean_dev[:b], self.std_dev[:b], self.ws,
                                 stream=torch.cuda.current_stream(self.model.device))
 
     def buckets(self):
-        out, b = [], 256
-        while b < self.capacity:
-            out.append(b)
-            b *= 4
-        out.append(self.capacity)
-        return out
+        """Batch sizes with a captured graph: 256 * sqrt(2)^i rounded up to a
+        multiple of 64 (one wave of queries), then the capacity, so a batch is
+        padded by at most ~1.41x."""
+        if getattr(self, "_buckets", None) is None:
+            out, i = [], 0
+            while True:
+                b = -(-int(round(256 * 2.0 ** (i / 2.0))) // 64) * 64
+                if b >= self.capacity:
+                    break
+                if not out or b > out[-1]:
+                    out.append(b)
+                i += 1
+            out.append(self.capacity)
+            self._buckets = out
+        return self._buckets
 
     def capture_all(self):
         """Capture the prediction graph of every bucket size (before serving starts:
