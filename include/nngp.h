@@ -146,13 +146,17 @@ int nngp_knn_rescale(const double* ref, int64_t n_ref, const double* query, int6
  * process-wide, read when the refs are prepared (at each nngp_knn call, at
  * nngp_model_create).  AUTO: a uniform-grid exact search for d <= 3 and k <= 64
  * (the lowest Vecchia rows scanned brute force), the brute-force scans
- * otherwise.  BRUTE_FORCE: always the scans.  Every method returns the same
+ * otherwise.  BRUTE_FORCE: always the scans.  In both, the FP32_REFINED screen
+ * of d >= 4 runs on the matrix cores (bf16 hi/lo split products, FP32
+ * accumulation, its own certified error bound).  BRUTE_FORCE_VALU: the scans,
+ * with the FP32 VALU screen for every d.  Every method returns the same
  * indices.  No reference counterpart (the reference has the brute-force scan
  * only, cuda_kernel_accelerators.py:663-745).
  */
 typedef enum {
     NNGP_KNN_AUTO = 0,
-    NNGP_KNN_BRUTE_FORCE = 1
+    NNGP_KNN_BRUTE_FORCE = 1,
+    NNGP_KNN_BRUTE_FORCE_VALU = 2
 } nngp_knn_method;
 int nngp_knn_set_method(int method);
 int nngp_knn_get_method(void);

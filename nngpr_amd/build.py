@@ -29,8 +29,11 @@ def build(quiet=False, jobs=None):
     """Compile every .hip source to an object, then link the shared library."""
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
+    # -amdgpu-mfma-vgpr-form: matrix instructions write their results to VGPRs, so
+    # the screen reads them without an AGPR-to-VGPR copy per result
     flags = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-fno-slp-vectorize",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-mllvm", "-amdgpu-mfma-vgpr-form",
              "-Wno-unused-command-line-argument", "-I", INCLUDE]
     procs = []
     objs = []

@@ -711,7 +711,7 @@ int nngp_model_destroy(nngp_model* model) {
 static void carve_model_ws(Carver& c, const nngp_model* M, int64_t nq, WsHeader** hdr, char** kmem,
                            size_t* kb, int32_t** nn, VecLayout* L) {
     *hdr = c.take<WsHeader>(1);
-    *kb = knn_search_bytes(M->n, nq, M->d, M->m, M->precision, false, false, M->refs.use_grid != 0);
+    *kb = knn_search_bytes(M->n, nq, M->d, M->m, M->precision, false, false, M->refs.layout);
     *kmem = c.take<char>(*kb);
     *nn = c.take<int32_t>((size_t)nq * M->m + 1);
     *L = carve_vec(c, nq, M->m, M->d, 0, false, M->precision);

@@ -98,17 +98,27 @@ int grid_search(const GridRefMem& G, const double* ru64, int64_t n_ref, int d,
 // ---------------------------------------------------------------------------
 // Ref-side state of the kNN search
 // ---------------------------------------------------------------------------
-// Scaled coordinates (AoS), and either the uniform grid (d <= 3 by default) or
-// the blocked scan layout (FP32 centred + norms for the screen, or FP64), with
-// the centre and max radius.
+// Search layout of a ref set, chosen from d, the precision and the process-wide
+// method when the refs are prepared:
+//   kRefGrid: the uniform grid (AUTO, d <= 3);
+//   kRefMfma: the bf16x3 matrix-core screen's operand fragments (FP32_REFINED,
+//             d >= 4, unless the method is BRUTE_FORCE_VALU);
+//   kRefScan: the blocked layouts of the VALU scans (FP32 screen or FP64 exact).
+enum RefLayout { kRefScan = 0, kRefGrid = 1, kRefMfma = 2 };
+int refs_layout(int d, int precision);
+
+// Scaled coordinates (AoS), and the layout the search reads, with the screen
+// centre and max radius.
 struct KnnRefs {
     const double* u64;
     const double* s64;
     const float* s32;
+    const uint16_t* frag;        // kRefMfma: ref operand fragments
     const double* centre;        // screen centre (mean of the finite scaled refs)
     const unsigned int* smax;
     GridRefMem grid;
-    int use_grid;
+    int layout;                  // RefLayout
+    int use_grid;                // layout == kRefGrid
     int64_t n;
     int d;
     int precision;
@@ -125,10 +135,10 @@ size_t knn_refs_bytes(int64_t n_ref, int d, int precision);
 int knn_prepare_refs(const double* ref, int64_t n_ref, int d, const double* scale, int n_scale,
                      int precision, void* mem, size_t bytes, KnnRefs* out, hipStream_t st);
 // with_cut: budget for searches with a Vecchia cut (nngp_knn); the serving
-// model's query searches have none.  grid_refs: the refs use the uniform grid
-// (KnnRefs::use_grid), so no scan buffer is budgeted when k fits the grid.
+// model's query searches have none.  layout: the refs' RefLayout (only the
+// buffers of the search that layout runs are budgeted).
 size_t knn_search_bytes(int64_t n_ref, int64_t n_query, int d, int k, int precision, bool same,
-                        bool with_cut, bool grid_refs);
+                        bool with_cut, int layout);
 // query == NULL searches the ref set itself (Vecchia self-search).
 int knn_search(const KnnRefs& R, const double* query, int64_t n_query, int k, int has_cut,
                int64_t ref_cut, int32_t* out_idx, void* ws, size_t ws_bytes, WsHeader* hdr,
@@ -156,6 +166,30 @@ size_t exact_rows_scratch_count(int k);
 void launch_exact_rows(const double* ru64, const double* qu64, int d, int64_t n_ref,
                        int64_t n_query, int has_cut, int64_t cut, int k, const ExactRows& er,
                        int32_t* out_idx, hipStream_t st);
+
+// ---------------------------------------------------------------------------
+// bf16x3 matrix-core screen of the brute-force search, d >= 4 (knn_mfma.hip)
+// ---------------------------------------------------------------------------
+bool mfma_screen_supported(int d);
+// Candidates per row for k neighbours (k + 8 to k + 23, one of 24, 40, 56, 72).
+int mfma_screen_K(int k);
+// bf16 entries of the operand fragments of n points (two padding tiles included).
+size_t mfma_frag_elems(int64_t n, int d);
+// Relative error bound of a screen score for the re-rank certificate.
+double mfma_screen_eps(int d);
+// Operand fragments of points [0, n) from their scaled coordinates u64: role 0 =
+// refs (also raises *smax, the largest centred norm), role 1 = queries (also
+// writes p2 = |u - centre|^2).
+void mfma_prep(const double* u64, int64_t n, int d, const double* centre, int role,
+               uint16_t* frag, double* p2, unsigned int* smax, hipStream_t st);
+// Screen of n_query rows (K candidates each): writes cand / thr and returns 0, or,
+// with ref stripes (spk given and the rows too few to fill the chip), writes
+// sorted per-stripe lists to spk and returns their count (a stripe merge
+// follows); -1 if K or d is not supported.  spk holds at least
+// min(8 n_query, 131072) x K entries.
+int launch_mfma_screen(const uint16_t* rfrag, int64_t n_ref, const uint16_t* qfrag,
+                       int64_t n_query, int d, int K, int has_cut, int64_t cut, int32_t* cand,
+                       float* thr, uint64_t* spk, hipStream_t st);
 
 // ---------------------------------------------------------------------------
 // Exact lists at a new scale from exact lists at an old one (knn_rescale.hip)

@@ -203,33 +203,6 @@ struct ScanArgs {
 };
 
 
-// Order-preserving map of an FP32 score onto uint32, packed with the ref index:
-// comparing the packed uint64 values orders by (score, index) and puts empty
-// slots (+inf, index -1) last.
-__device__ __forceinline__ uint32_t ord_f32(float f) {
-    const uint32_t b = __float_as_uint(f);
-    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
-}
-__device__ __forceinline__ float unord_f32(uint32_t u) {
-    return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
-}
-__device__ __forceinline__ uint64_t pack_key(float s, int idx) {
-    return ((uint64_t)ord_f32(s) << 32) | (uint32_t)idx;
-}
-
-template <int K>
-__device__ __forceinline__ void topk_insert_packed(uint64_t (&e)[K], uint64_t x) {
-    // Screening needs the candidate set, not an order among equal scores: only
-    // the score half is compared (one 32-bit compare and four selects per step).
-#pragma unroll
-    for (int t = 0; t < K; ++t) {
-        const bool lt = (uint32_t)(x >> 32) < (uint32_t)(e[t] >> 32);
-        const uint64_t lo = lt ? x : e[t];
-        x = lt ? e[t] : x;
-        e[t] = lo;
-    }
-}
-
 // Score of ref u of an 8-ref block for one lane's query.
 //   screen: -thr + sum_f (-2 q_f) r_f, compared against -|r|^2 (feature D);
 //   exact:  sum_f (r_f - q_f)^2 with the FMA order of the reference key.
@@ -298,7 +271,7 @@ __global__ __launch_bounds__(64) void scan_kernel(ScanArgs<T> a) {
     }
     // Empty slot: (+inf, index -1); its unpacked score is +inf, never NaN.
 #pragma unroll
-    for (int t = 0; t < KP; ++t) pk[t] = ((uint64_t)0xFF800000u << 32) | 0xFFFFFFFFu;
+    for (int t = 0; t < KP; ++t) pk[t] = kPackedEmpty;
     // Admission threshold: screen compares score - thr < -|r|^2, exact key < thr.
     T thr = qv ? dev_inf<T>() : -dev_inf<T>();
     T negthr = -thr;
@@ -942,6 +915,21 @@ int launch_rerank(const RerankArgs& a, int K, hipStream_t st) {
     }
 }
 
+// Merge of n_lists sorted screen lists per row (the matrix-core screen's ref
+// stripe halves) into candidates and K-th scores.
+int launch_stripe_merge(int K, const uint64_t* pk, int n_lists, int64_t n_query, int32_t* cand,
+                        float* thr, hipStream_t st) {
+    if (n_lists > kMaxStripes) return -1;
+    const dim3 g((unsigned)((n_query + 255) / 256));
+    switch (K) {
+        case 24: hipLaunchKernelGGL(stripe_merge_kernel<24>, g, dim3(256), 0, st, pk, n_lists, n_query, cand, thr); return 0;
+        case 40: hipLaunchKernelGGL(stripe_merge_kernel<40>, g, dim3(256), 0, st, pk, n_lists, n_query, cand, thr); return 0;
+        case 56: hipLaunchKernelGGL(stripe_merge_kernel<56>, g, dim3(256), 0, st, pk, n_lists, n_query, cand, thr); return 0;
+        case 72: hipLaunchKernelGGL(stripe_merge_kernel<72>, g, dim3(256), 0, st, pk, n_lists, n_query, cand, thr); return 0;
+        default: return -1;
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -956,11 +944,13 @@ struct RefMem {
     double* u64;
     double* s64;
     float* s32;
+    uint16_t* frag;
     GridRefMem grid;
 };
 
-// Both the scan layout and (d <= 3) the grid are budgeted, so that one size
-// serves either search method.
+// Every layout a method could choose for these refs is budgeted (the scan
+// layout, the grid for d <= 3, the matrix-core fragments for d >= 4 in
+// FP32_REFINED), so that one size serves every search method.
 RefMem carve_refs(Carver& c, int64_t n_ref, int d, int precision) {
     RefMem M{};
     const int64_t rpad = round_up(std::max<int64_t>(n_ref, 1), 64) + 64;
@@ -973,6 +963,8 @@ RefMem carve_refs(Carver& c, int64_t n_ref, int d, int precision) {
         if (precision == NNGP_PRECISION_FP32_REFINED) M.s32 = c.take<float>((size_t)rpad * (dp + 1));
         else M.s64 = c.take<double>((size_t)rpad * dp);
     }
+    if (precision == NNGP_PRECISION_FP32_REFINED && mfma_screen_supported(d))
+        M.frag = c.take<uint16_t>(mfma_frag_elems(n_ref, d));
     if (d <= 3) carve_grid_refs(c, n_ref, &M.grid);
     return M;
 }
@@ -981,6 +973,7 @@ struct QueryMem {
     double* u64;
     double* s64;
     float* s32;
+    uint16_t* qfrag;  // matrix-core screen: query operand fragments
     double* p2;
     int32_t* cand;
     float* thr;
@@ -998,25 +991,42 @@ struct QueryMem {
     GridSearchMem grid;
 };
 
-// Query-side buffers of a search.  grid_refs: the refs use the uniform grid and
-// k fits it, so the scans never run and none of their buffers is carved.
+// Which search a knn_search over refs of this layout runs for k neighbours.
+struct SearchKind {
+    bool grid;    // uniform-grid search
+    bool mfma;    // matrix-core screen + re-rank
+    bool fast;    // a streaming scan (screen or exact)
+    bool screen;  // a screen (VALU or matrix-core) + re-rank
+};
+SearchKind search_kind(int layout, int d, int k, int precision) {
+    SearchKind s{};
+    s.grid = layout == kRefGrid && grid_supported(d, k);
+    s.mfma = !s.grid && layout == kRefMfma && k >= 1 && k <= 64;
+    s.fast = s.mfma || (!s.grid && layout != kRefMfma && use_fast_scan(d, k));
+    s.screen = s.mfma || (s.fast && precision == NNGP_PRECISION_FP32_REFINED);
+    return s;
+}
+
+// Query-side buffers of a search over refs of the given layout: only the
+// buffers of the search it runs for k (search_kind) are carved.
 QueryMem carve_query(Carver& c, int64_t n_ref, int64_t n_query, int d, int k, int precision,
-                     bool same, bool with_cut, bool grid_refs) {
+                     bool same, bool with_cut, int layout) {
     QueryMem Q{};
-    if (grid_refs && grid_supported(d, k)) {
+    const SearchKind sk = search_kind(layout, d, k, precision);
+    if (sk.grid) {
         if (!same) Q.u64 = c.take<double>((size_t)n_query * d + 1);
         carve_grid_search(c, n_ref, n_query, k, with_cut, &Q.grid);
         return Q;
     }
     const int64_t qpad = round_up(std::max<int64_t>(n_query, 1), 64) + 64;
     const int dp = scan_dim(d);
-    const bool fast = use_fast_scan(d, k);
-    const bool screen = fast && precision == NNGP_PRECISION_FP32_REFINED;
+    const bool fast = sk.fast, screen = sk.screen;
     if (!same) {
         Q.u64 = c.take<double>((size_t)n_query * d + 1);
         if (fast && !screen) Q.s64 = c.take<double>((size_t)qpad * dp);
-        if (screen) Q.s32 = c.take<float>((size_t)qpad * (dp + 1));
+        if (screen && !sk.mfma) Q.s32 = c.take<float>((size_t)qpad * (dp + 1));
     }
+    if (sk.mfma) Q.qfrag = c.take<uint16_t>(mfma_frag_elems(n_query, d));
     if (fast && !screen && n_ref >= 2 * 2048) {  // exact-scan stripe lists (bound as below)
         const size_t ne = (size_t)std::min<int64_t>(8 * n_query, 2048 * 64) * exact_K(k) + 1;
         Q.sxk = c.take<double>(ne);
@@ -1024,7 +1034,7 @@ QueryMem carve_query(Carver& c, int64_t n_ref, int64_t n_query, int d, int k, in
     }
     if (screen) {
         Q.p2 = c.take<double>((size_t)n_query + 1);
-        const int K = screen_K(k);
+        const int K = sk.mfma ? mfma_screen_K(k) : screen_K(k);
         Q.cand = c.take<int32_t>((size_t)n_query * K + 1);
         Q.thr = c.take<float>((size_t)n_query + 1);
         // stripe lists: stripes x rows <= min(8 n_query, 2048 x 64) for every
@@ -1042,10 +1052,6 @@ QueryMem carve_query(Carver& c, int64_t n_ref, int64_t n_query, int d, int k, in
     Q.lidx = c.take<int32_t>((size_t)k * kFallbackThreads);
     return Q;
 }
-
-// Whether refs of d features prepared now would use the grid (the search method
-// is read when refs are prepared).
-bool refs_would_use_grid(int d) { return d <= 3 && knn_method() == NNGP_KNN_AUTO; }
 
 std::atomic<int> g_knn_method{NNGP_KNN_AUTO};
 
@@ -1102,9 +1108,22 @@ void launch_exact_rows(const double* ru64, const double* qu64, int d, int64_t n_
 int knn_method() { return g_knn_method.load(); }
 
 int knn_set_method(int method) {
-    if (method != NNGP_KNN_AUTO && method != NNGP_KNN_BRUTE_FORCE) return NNGP_ERR_INVALID_ARGUMENT;
+    if (method != NNGP_KNN_AUTO && method != NNGP_KNN_BRUTE_FORCE &&
+        method != NNGP_KNN_BRUTE_FORCE_VALU)
+        return NNGP_ERR_INVALID_ARGUMENT;
     g_knn_method.store(method);
     return NNGP_OK;
+}
+
+// The layout refs of d features prepared now get (the method is read when refs
+// are prepared).
+int refs_layout(int d, int precision) {
+    const int m = knn_method();
+    if (d <= 3 && m == NNGP_KNN_AUTO) return kRefGrid;
+    if (precision == NNGP_PRECISION_FP32_REFINED && mfma_screen_supported(d) &&
+        m != NNGP_KNN_BRUTE_FORCE_VALU)
+        return kRefMfma;
+    return kRefScan;
 }
 
 size_t knn_refs_bytes(int64_t n_ref, int d, int precision) {
@@ -1120,16 +1139,20 @@ int knn_prepare_refs(const double* ref, int64_t n_ref, int d, const double* scal
     Carver c(mem, bytes);
     RefMem M = carve_refs(c, n_ref, d, precision);
     if (!c.ok() || !mem) return NNGP_ERR_WORKSPACE_TOO_SMALL;
-    const bool grid = refs_would_use_grid(d);
-    if (grid) {  // the grid search needs the scaled coordinates only
+    const int layout = refs_layout(d, precision);
+    const bool grid = layout == kRefGrid;
+    if (layout != kRefScan) {  // the grid search and the matrix-core screen read neither
         M.s64 = nullptr;
         M.s32 = nullptr;
     }
+    if (layout != kRefMfma) M.frag = nullptr;
     KnnRefs R{};
     R.u64 = M.u64;
     R.s64 = M.s64;
     R.s32 = M.s32;
+    R.frag = M.frag;
     R.grid = M.grid;
+    R.layout = layout;
     R.use_grid = grid ? 1 : 0;
     R.centre = M.centre;
     R.smax = M.smax;
@@ -1150,7 +1173,7 @@ int knn_prepare_refs(const double* ref, int64_t n_ref, int d, const double* scal
     pr.d = d;
     pr.n_scale = n_scale;
     for (int f = 0; f < n_scale; ++f) pr.scale[f] = scale[f];
-    if (M.s32) {
+    if (M.s32 || M.frag) {
         hipLaunchKernelGGL(centre_partial_kernel, dim3(kCentreBlocks), dim3(256), 0, st, pr,
                            M.cpart);
         hipLaunchKernelGGL(centre_final_kernel, dim3(1), dim3(64), 0, st, M.cpart, kCentreBlocks,
@@ -1159,15 +1182,16 @@ int knn_prepare_refs(const double* ref, int64_t n_ref, int d, const double* scal
     const int64_t rpad = round_up(n_ref, 64) + 64;
     hipLaunchKernelGGL(prep_kernel, dim3((unsigned)((rpad + 255) / 256)), dim3(256), 0, st, pr,
                        M.centre, rpad, scan_dim(d), M.u64, M.s64, M.s32, (double*)nullptr, M.smax);
+    if (M.frag) mfma_prep(M.u64, n_ref, d, M.centre, 0, M.frag, nullptr, M.smax, st);
     NNGP_HIP_CHECK(hipGetLastError());
     if (grid) return grid_build(M.u64, n_ref, d, M.grid, st);
     return NNGP_OK;
 }
 
 size_t knn_search_bytes(int64_t n_ref, int64_t n_query, int d, int k, int precision, bool same,
-                        bool with_cut, bool grid_refs) {
+                        bool with_cut, int layout) {
     Carver c(nullptr, 0);
-    carve_query(c, n_ref, n_query, d, k, precision, same, with_cut, grid_refs);
+    carve_query(c, n_ref, n_query, d, k, precision, same, with_cut, layout);
     return c.used + 256;
 }
 
@@ -1177,9 +1201,10 @@ int knn_search(const KnnRefs& R, const double* query, int64_t n_query, int k, in
     const int d = R.d;
     if (k < 1 || k > NNGP_MAX_K || n_query < 0) return NNGP_ERR_INVALID_ARGUMENT;
     const bool same = query == nullptr;
-    const bool grid_path = R.use_grid && grid_supported(d, k);
+    const SearchKind sk = search_kind(R.layout, d, k, R.precision);
+    const bool grid_path = sk.grid;
     Carver c(ws, ws_bytes);
-    QueryMem Q = carve_query(c, R.n, n_query, d, k, R.precision, same, has_cut != 0, grid_path);
+    QueryMem Q = carve_query(c, R.n, n_query, d, k, R.precision, same, has_cut != 0, R.layout);
     if (!c.ok() || (!ws && c.used > 0)) return NNGP_ERR_WORKSPACE_TOO_SMALL;
     if (n_query == 0) return NNGP_OK;
     if (R.n == 0) {
@@ -1190,8 +1215,9 @@ int knn_search(const KnnRefs& R, const double* query, int64_t n_query, int k, in
         return NNGP_OK;
     }
     // the grid path reads only the query's scaled coordinates (no scan layouts)
-    const bool fast = !grid_path && use_fast_scan(d, k) && (R.s32 || R.s64);
-    const bool screen = fast && R.precision == NNGP_PRECISION_FP32_REFINED;
+    const bool mfma = sk.mfma;
+    const bool fast = sk.fast && (mfma || R.s32 || R.s64);
+    const bool screen = fast && sk.screen;
     const int dp = scan_dim(d);
     const int64_t qpad = round_up(n_query, 64) + 64;
     const double* qu64 = R.u64;
@@ -1207,19 +1233,21 @@ int knn_search(const KnnRefs& R, const double* query, int64_t n_query, int k, in
     if (!same) {
         // only the layouts the chosen search reads are written
         double* qs64w = fast && !screen ? Q.s64 : nullptr;
-        float* qs32w = screen ? Q.s32 : nullptr;
+        float* qs32w = screen && !mfma ? Q.s32 : nullptr;
         hipLaunchKernelGGL(prep_kernel, dim3((unsigned)((qpad + 255) / 256)), dim3(256), 0, st, pr,
-                           R.centre, qpad, dp, Q.u64, qs64w, qs32w, screen ? Q.p2 : nullptr,
-                           (unsigned int*)nullptr);
+                           R.centre, qpad, dp, Q.u64, qs64w, qs32w,
+                           screen && !mfma ? Q.p2 : nullptr, (unsigned int*)nullptr);
         qu64 = Q.u64;
         qs64 = qs64w;
         qs32 = qs32w;
-    } else if (screen) {
+    } else if (screen && !mfma) {
         // |u_q - centre|^2 of the refs themselves (the query set is the ref set)
         pr.x = nullptr;
         hipLaunchKernelGGL(p2_kernel, dim3((unsigned)((n_query + 255) / 256)), dim3(256), 0, st,
                            R.u64, n_query, d, R.centre, Q.p2);
     }
+    // query operand fragments (and |u_q - centre|^2) of the matrix-core screen
+    if (mfma) mfma_prep(qu64, n_query, d, R.centre, 1, Q.qfrag, Q.p2, nullptr, st);
     NNGP_HIP_CHECK(hipGetLastError());
 
     if (grid_path) {
@@ -1287,27 +1315,37 @@ int knn_search(const KnnRefs& R, const double* query, int64_t n_query, int k, in
         NNGP_HIP_CHECK(hipGetLastError());
         return NNGP_OK;
     }
-    const int K = screen_K(k);
-    ScanArgs<float> a{};
-    a.ref = R.s32;
-    a.n_ref = (int32_t)R.n;
-    a.qry = qs32;
-    a.n_query = n_query;
-    a.has_cut = has_cut;
-    a.cut = ref_cut;
-    a.reverse = has_cut;
-    a.out_idx = Q.cand;
-    a.out_thr = Q.thr;
-    const int n_stripes = screen_stripes(R.n, n_query);
-    if (n_stripes > 1 && Q.spk) {
-        a.stripe_w = screen_stripe_w(R.n, n_stripes);
-        a.out_pk = Q.spk;
-    }
+    const int K = mfma ? mfma_screen_K(k) : screen_K(k);
     hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, Q.flag_count, 1);
-    timing_begin(TK_KNN_SCAN, st);
-    const int rcs = launch_scan<float, true>(a, d, K, st);
-    timing_end(TK_KNN_SCAN, st);
-    if (rcs != 0) return NNGP_ERR_UNSUPPORTED;
+    if (mfma) {
+        timing_begin(TK_KNN_SCAN, st);
+        const int lists = launch_mfma_screen(R.frag, R.n, Q.qfrag, n_query, d, K, has_cut, ref_cut,
+                                             Q.cand, Q.thr, Q.spk, st);
+        if (lists > 0 && launch_stripe_merge(K, Q.spk, lists, n_query, Q.cand, Q.thr, st) != 0)
+            return NNGP_ERR_UNSUPPORTED;
+        timing_end(TK_KNN_SCAN, st);
+        if (lists < 0) return NNGP_ERR_UNSUPPORTED;
+    } else {
+        ScanArgs<float> a{};
+        a.ref = R.s32;
+        a.n_ref = (int32_t)R.n;
+        a.qry = qs32;
+        a.n_query = n_query;
+        a.has_cut = has_cut;
+        a.cut = ref_cut;
+        a.reverse = has_cut;
+        a.out_idx = Q.cand;
+        a.out_thr = Q.thr;
+        const int n_stripes = screen_stripes(R.n, n_query);
+        if (n_stripes > 1 && Q.spk) {
+            a.stripe_w = screen_stripe_w(R.n, n_stripes);
+            a.out_pk = Q.spk;
+        }
+        timing_begin(TK_KNN_SCAN, st);
+        const int rcs = launch_scan<float, true>(a, d, K, st);
+        timing_end(TK_KNN_SCAN, st);
+        if (rcs != 0) return NNGP_ERR_UNSUPPORTED;
+    }
 
     RerankArgs rr{};
     rr.ru64 = R.u64;
@@ -1324,7 +1362,7 @@ int knn_search(const KnnRefs& R, const double* query, int64_t n_query, int k, in
     rr.flag_count = Q.flag_count;
     rr.flag_key = Q.flag_key;
     rr.flag_idx = Q.flag_idx;
-    rr.eps_factor = (4.0 * d + 16.0) * std::ldexp(1.0, -24);
+    rr.eps_factor = mfma ? mfma_screen_eps(d) : (4.0 * d + 16.0) * std::ldexp(1.0, -24);
     timing_begin(TK_KNN_RERANK, st);
     const int rcr = launch_rerank(rr, K, st);
     timing_end(TK_KNN_RERANK, st);
@@ -1350,9 +1388,9 @@ size_t knn_workspace_bytes(int64_t n_ref, int64_t n_query, int d, int k, int pre
                            bool same) {
     Carver c(nullptr, 0);
     c.take<WsHeader>(1);
-    const bool grid = refs_would_use_grid(d);
     return c.used + knn_refs_bytes(n_ref, d, precision) +
-           knn_search_bytes(n_ref, n_query, d, k, precision, same, true, grid) + 512;
+           knn_search_bytes(n_ref, n_query, d, k, precision, same, true,
+                            refs_layout(d, precision)) + 512;
 }
 
 int knn_run(const double* ref, int64_t n_ref, const double* query, int64_t n_query, int d,
@@ -1370,7 +1408,7 @@ int knn_run(const double* ref, int64_t n_ref, const double* query, int64_t n_que
     // sized for the method read now; knn_search carves for the refs' actual
     // method and reports a workspace too small if the method changed in between
     const size_t qb = knn_search_bytes(n_ref, n_query, d, k, precision, same, has_cut != 0,
-                                       refs_would_use_grid(d));
+                                       refs_layout(d, precision));
     char* qmem = c.take<char>(qb);
     if (!c.ok()) return NNGP_ERR_WORKSPACE_TOO_SMALL;
     if (reset_stats) hipLaunchKernelGGL(zero_header_kernel, dim3(1), dim3(64), 0, st, hdr);

@@ -29,6 +29,36 @@ __device__ __forceinline__ int32_t wave_max32(int32_t v) {
     return __builtin_amdgcn_readfirstlane(v);
 }
 
+// Order-preserving map of an FP32 score onto uint32, packed with the ref index:
+// comparing the packed uint64 values orders by (score, index) and puts empty
+// slots (+inf, index -1) last.
+__device__ __forceinline__ uint32_t ord_f32(float f) {
+    const uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float unord_f32(uint32_t u) {
+    return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+__device__ __forceinline__ uint64_t pack_key(float s, int idx) {
+    return ((uint64_t)ord_f32(s) << 32) | (uint32_t)idx;
+}
+// Empty slot of a packed screen list: score +inf, index -1.
+constexpr uint64_t kPackedEmpty = ((uint64_t)0xFF800000u << 32) | 0xFFFFFFFFu;
+
+// Insertion into a packed screen list sorted by score.  Screening needs the
+// candidate set, not an order among equal scores: only the score half is
+// compared (one 32-bit compare and four selects per step).
+template <int K>
+__device__ __forceinline__ void topk_insert_packed(uint64_t (&e)[K], uint64_t x) {
+#pragma unroll
+    for (int t = 0; t < K; ++t) {
+        const bool lt = (uint32_t)(x >> 32) < (uint32_t)(e[t] >> 32);
+        const uint64_t lo = lt ? x : e[t];
+        x = lt ? e[t] : x;
+        e[t] = lo;
+    }
+}
+
 // Compare-exchange ladder over a sorted register list, ordered by (key, index):
 // the carried element moves ahead of an equal key with a higher index.  Indices
 // compare unsigned, so an empty slot (index -1) sorts after every real entry of

@@ -15,6 +15,7 @@ PRECISION_FP64 = nat.PRECISION_FP64
 PRECISION_FP32_REFINED = nat.PRECISION_FP32_REFINED
 KNN_AUTO = nat.KNN_AUTO
 KNN_BRUTE_FORCE = nat.KNN_BRUTE_FORCE
+KNN_BRUTE_FORCE_VALU = nat.KNN_BRUTE_FORCE_VALU
 
 _workspaces = {}
 
