@@ -528,83 +528,99 @@ struct RerankArgs {
 };
 
 
-template <int K>
-__global__ __launch_bounds__(256) void rerank_kernel(RerankArgs a) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= a.n_query) return;
-    double key[K];
-    int ids[K];
-    int count = 0;   // candidates the screen kept
-    int nfin = 0;    // of which with a finite exact key (the others are never neighbours)
-    const double* uq = a.qu64 + q * a.d;
-#pragma unroll
-    for (int t = 0; t < K; ++t) {
-        int id = a.cand[q * K + t];
-        key[t] = __builtin_inf();
-        if (id >= 0) {
-            key[t] = exact_key(a.ru64 + (int64_t)id * a.d, uq, a.d);
-            ++count;
-            if (!(key[t] < __builtin_inf())) id = -1;
-        }
-        ids[t] = id;
-        nfin += id >= 0 ? 1 : 0;
-    }
-    // m-th (key, index) pair; +inf when fewer than m finite keys (never certified)
-    double dm = __builtin_inf();
-    int dmi = -1;
-#pragma unroll
-    for (int t = 0; t < K; ++t) {
-        if (ids[t] < 0) continue;
-        int rank = 0;
-#pragma unroll
-        for (int s = 0; s < K; ++s) {
-            bool before = ids[s] >= 0 &&
-                          (key[s] < key[t] || (key[s] == key[t] && ids[s] < ids[t]));
-            rank += before ? 1 : 0;
-        }
-        if (rank < a.k) a.out_idx[q * a.k + rank] = ids[t];
-        if (rank == a.k - 1) {
-            dm = key[t];
-            dmi = ids[t];
-        }
-    }
-    for (int j = nfin; j < a.k; ++j) a.out_idx[q * a.k + j] = -1;
-    if (count < K) return;  // every ref of the row was a candidate: exact
-    // Certificate.  A non-candidate r, of centred norm rho (the larger of its exact
-    // and FP32-rounded norms, rho <= S, the largest over the refs), had FP32 score
-    // >= thr, so its exact key is (the FP32 rounding of the screen's score and of
-    // its threshold comparison, E = eps_factor)
-    //   >= f(rho) = thr + |p|^2 - (E + 1e-12) (|thr| + (|p| + rho)^2),
-    // and by the reverse triangle inequality (the exact norm is >= rho / 1.0002)
-    //   >= g(rho) = (rho / 1.0002 - |p|)^2 - 1e-12 (|p| + rho)^2  for rho >= 1.0002 |p|.
-    // f decreases, and g increases on [1.0003 |p|, inf), so for any rc in
-    // [1.0003 |p|, S) every non-candidate key is >= min(f(rc), g(rc)) (rc is taken
-    // at the crossing of f and g); if the crossing or 1.0003 |p| is >= S it is >= f(S).  Far refs then
-    // raise S without loosening the bound of the rows near the centre.  Strictly
-    // above the m-th candidate key => exact top-m.
+// Certificate of a screened row whose K candidates were all kept.  A
+// non-candidate r, of centred norm rho (the larger of its exact and FP32-rounded
+// norms, rho <= S, the largest over the refs), had a screen score >= thr, so its
+// exact key is (the rounding of the screen's score and of its threshold
+// comparison, E = eps_factor)
+//   >= f(rho) = thr + |p|^2 - (E + 1e-12) (|thr| + (|p| + rho)^2),
+// and by the reverse triangle inequality (the exact norm is >= rho / 1.0002)
+//   >= g(rho) = (rho / 1.0002 - |p|)^2 - 1e-12 (|p| + rho)^2  for rho >= 1.0002 |p|.
+// f decreases, and g increases on [1.0003 |p|, inf), so for any rc in
+// [1.0003 |p|, S) every non-candidate key is >= min(f(rc), g(rc)) (rc is taken
+// at the crossing of f and g); if the crossing or 1.0003 |p| is >= S it is >= f(S).
+// Far refs then raise S without loosening the bound of the rows near the centre.
+// The row's top m is exact when its m-th candidate key is strictly below this.
+__device__ __forceinline__ double rerank_bound(const RerankArgs& a, int64_t q) {
     const double s = (double)__uint_as_float(*a.smax_bits);
     const double p2 = a.qp2[q];
     const double p = sqrt(p2);
     const double cc = a.eps_factor + 1e-12;
     const double A = (double)a.thr[q] + p2 - cc * fabs((double)a.thr[q]);
     const double rlo = 1.0003 * p;
-    double bound;
-    if (!(s > rlo)) {
-        bound = A - cc * (p + s) * (p + s);
-    } else {
-        // f(r) = (r - p)^2  <=>  (1 + cc) r^2 - 2 p (1 - cc) r + (1 + cc) p^2 - A = 0
-        const double disc = (1.0 + cc) * A - 4.0 * cc * p2;
-        double rc = disc > 0.0 ? (p * (1.0 - cc) + sqrt(disc)) / (1.0 + cc) : rlo;
-        rc = fmax(rc, rlo);
-        if (!(rc < s)) {  // every ref has rho <= rc: f alone
-            bound = A - cc * (p + s) * (p + s);
-        } else {
-            const double t = rc / 1.0002 - p;
-            const double m2 = (p + rc) * (p + rc);
-            bound = fmin(A - cc * m2, t * t - 1e-12 * m2);
+    if (!(s > rlo)) return A - cc * (p + s) * (p + s);
+    // f(r) = (r - p)^2  <=>  (1 + cc) r^2 - 2 p (1 - cc) r + (1 + cc) p^2 - A = 0
+    const double disc = (1.0 + cc) * A - 4.0 * cc * p2;
+    double rc = disc > 0.0 ? (p * (1.0 - cc) + sqrt(disc)) / (1.0 + cc) : rlo;
+    rc = fmax(rc, rlo);
+    if (!(rc < s)) return A - cc * (p + s) * (p + s);  // every ref has rho <= rc: f alone
+    const double t = rc / 1.0002 - p;
+    const double m2 = (p + rc) * (p + rc);
+    return fmin(A - cc * m2, t * t - 1e-12 * m2);
+}
+
+// One wave per row: lane j re-keys candidates j and j + 64 in FP64 (the exact
+// key, gathers of all candidates in flight together), ranks its candidates among
+// all K by (key, index) through shuffles, writes the top k and certifies the row;
+// uncertified rows are listed with their m-th pair for the exact recompute.
+template <int K>
+__global__ __launch_bounds__(256) void rerank_kernel(RerankArgs a) {
+    constexpr int J = (K + 63) / 64;   // candidate slots per lane
+    const int lane = threadIdx.x & 63;
+    const int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (q >= a.n_query) return;        // uniform over the wave
+    const double* uq = a.qu64 + q * a.d;
+    double key[J];
+    int ids[J];
+    int count = 0;  // candidates the screen kept
+    int nfin = 0;   // of which with a finite exact key (the others are never neighbours)
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int t = lane + 64 * j;
+        int id = t < K ? a.cand[q * K + t] : -1;
+        count += __popcll(__ballot(id >= 0));
+        key[j] = __builtin_inf();
+        if (id >= 0) {
+            key[j] = exact_key(a.ru64 + (int64_t)id * a.d, uq, a.d);
+            if (!(key[j] < __builtin_inf())) id = -1;
+        }
+        ids[j] = id;
+        nfin += __popcll(__ballot(id >= 0));
+    }
+    int rank[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) rank[j] = 0;
+#pragma unroll
+    for (int js = 0; js < J; ++js) {
+        const int smax = js + 1 < J ? 64 : K - 64 * js;
+#pragma unroll 4
+        for (int sl = 0; sl < smax; ++sl) {
+            const double ks = __shfl(key[js], sl, 64);
+            const int is = __shfl(ids[js], sl, 64);
+#pragma unroll
+            for (int j = 0; j < J; ++j) {
+                const bool before = is >= 0 && (ks < key[j] || (ks == key[j] && is < ids[j]));
+                rank[j] += before ? 1 : 0;
+            }
         }
     }
-    if (!(dm < bound)) {
+    // m-th (key, index) pair; +inf when fewer than m finite keys (never certified)
+    double dm = __builtin_inf();
+    int dmi = -1;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        if (ids[j] >= 0 && rank[j] < a.k) a.out_idx[q * a.k + rank[j]] = ids[j];
+        const uint64_t m = __ballot(ids[j] >= 0 && rank[j] == a.k - 1);
+        if (m) {
+            const int src = __ffsll((unsigned long long)m) - 1;
+            dm = __shfl(key[j], src, 64);
+            dmi = __shfl(ids[j], src, 64);
+        }
+    }
+    for (int jj = nfin + lane; jj < a.k; jj += 64) a.out_idx[q * a.k + jj] = -1;
+    if (count < K) return;  // every ref of the row was a candidate: exact
+    const double bound = rerank_bound(a, q);
+    if (lane == 0 && !(dm < bound)) {
         unsigned long long slot = atomicAdd(a.flag_count, 1ULL);
         a.flag_rows[slot] = q;
         a.flag_key[slot] = dm;
@@ -897,7 +913,7 @@ int launch_scan(const ScanArgs<T>& a, int d, int K, hipStream_t st) {
 
 template <int K>
 void launch_rerank_k(const RerankArgs& a, hipStream_t st) {
-    const unsigned grid = (unsigned)((a.n_query + 255) / 256);
+    const unsigned grid = (unsigned)((a.n_query + 3) / 4);   // one wave per row
     hipLaunchKernelGGL(rerank_kernel<K>, dim3(grid), dim3(256), 0, st, a);
 }
 
