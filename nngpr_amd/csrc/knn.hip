@@ -629,11 +629,12 @@ __global__ __launch_bounds__(256) void rerank_kernel(RerankArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Wave-parallel exact recomputation of flagged rows.  One wave per row: lanes
-// stream the row's refs, keep every (key, index) <= the row's bound pair in LDS,
-// then rank the kept entries.  The true top-k pairs are all <= the bound, so they
-// are all kept.  Rows that keep more than CAP entries (massive exact ties) go to
-// the per-row serial kernel instead.
+// Exact recomputation of flagged rows.  One block per row: its threads stream the
+// row's refs (each keying kFbRefs refs per step, their loads in flight together),
+// keep every (key, index) <= the row's bound pair in LDS, then rank the kept
+// entries.  The true top-k pairs are all <= the bound, so they are all kept.  Rows
+// that keep more than kFbCap entries (massive exact ties) go to the per-row serial
+// kernel instead.
 // ---------------------------------------------------------------------------
 struct WaveFbArgs {
     const double* ru64;
@@ -653,17 +654,16 @@ struct WaveFbArgs {
 };
 
 constexpr int kFbCap = 1024;
-constexpr int kFbWaves = 4;
+constexpr int kFbThreads = 256;
+constexpr int kFbRefs = 4;   // refs each thread keys per step (their loads in flight together)
 
-__global__ __launch_bounds__(256) void wave_fallback_kernel(WaveFbArgs a) {
-    __shared__ double skey[kFbWaves][kFbCap];
-    __shared__ int sidx[kFbWaves][kFbCap];
-    __shared__ int scnt[kFbWaves];
-    const int w = threadIdx.x / 64;
-    const int lane = threadIdx.x % 64;
+__global__ __launch_bounds__(kFbThreads) void row_fallback_kernel(WaveFbArgs a) {
+    __shared__ double skey[kFbCap];
+    __shared__ int sidx[kFbCap];
+    __shared__ int scnt;
+    const int tid = threadIdx.x;
     const int64_t nrows = (int64_t)*a.row_count;
-    for (int64_t slot = (int64_t)blockIdx.x * kFbWaves + w; slot < nrows;
-         slot += (int64_t)gridDim.x * kFbWaves) {
+    for (int64_t slot = blockIdx.x; slot < nrows; slot += gridDim.x) {
         const int64_t q = a.rows[slot];
         const double bk = a.bound_key[slot];
         const int bi = a.bound_idx[slot];
@@ -672,44 +672,50 @@ __global__ __launch_bounds__(256) void wave_fallback_kernel(WaveFbArgs a) {
             lim = q + a.cut;
             lim = lim < 0 ? 0 : (lim > a.n_ref ? a.n_ref : lim);
         }
-        if (lane == 0) scnt[w] = 0;
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        if (tid == 0) scnt = 0;
+        __syncthreads();
         const double* uq = a.qu64 + q * a.d;
-        for (int64_t r = lane; r < lim; r += 64) {
-            const double key = exact_key(a.ru64 + r * a.d, uq, a.d);
-            if (key < bk || (key == bk && (int)r <= bi)) {
-                const int pos = atomicAdd(&scnt[w], 1);
-                if (pos < kFbCap) {
-                    skey[w][pos] = key;
-                    sidx[w][pos] = (int)r;
+        for (int64_t r0 = 0; r0 < lim; r0 += (int64_t)kFbThreads * kFbRefs) {
+            double key[kFbRefs];
+#pragma unroll
+            for (int u = 0; u < kFbRefs; ++u) {
+                const int64_t r = r0 + (int64_t)u * kFbThreads + tid;
+                key[u] = r < lim ? exact_key(a.ru64 + r * a.d, uq, a.d) : __builtin_inf();
+            }
+#pragma unroll
+            for (int u = 0; u < kFbRefs; ++u) {
+                const int64_t r = r0 + (int64_t)u * kFbThreads + tid;
+                if (r < lim && (key[u] < bk || (key[u] == bk && (int)r <= bi))) {
+                    const int pos = atomicAdd(&scnt, 1);
+                    if (pos < kFbCap) {
+                        skey[pos] = key[u];
+                        sidx[pos] = (int)r;
+                    }
                 }
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-        const int cnt = scnt[w];
+        __syncthreads();
+        const int cnt = scnt;
         if (cnt > kFbCap) {
-            if (lane == 0) {
+            if (tid == 0) {
                 unsigned long long o = atomicAdd(a.over_count, 1ULL);
                 a.over_rows[o] = q;
             }
         } else {
-            for (int e = lane; e < cnt; e += 64) {
-                const double ke = skey[w][e];
-                const int ie = sidx[w][e];
+            for (int e = tid; e < cnt; e += kFbThreads) {
+                const double ke = skey[e];
+                const int ie = sidx[e];
                 int rank = 0;
                 for (int f = 0; f < cnt; ++f) {
-                    const double kf = skey[w][f];
-                    const int ifx = sidx[w][f];
+                    const double kf = skey[f];
+                    const int ifx = sidx[f];
                     rank += (kf < ke || (kf == ke && ifx < ie)) ? 1 : 0;
                 }
                 if (rank < a.k) a.out_idx[q * a.k + rank] = ie;
             }
-            for (int j = cnt + lane; j < a.k; j += 64) a.out_idx[q * a.k + j] = -1;
+            for (int j = cnt + tid; j < a.k; j += kFbThreads) a.out_idx[q * a.k + j] = -1;
         }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-        __builtin_amdgcn_wave_barrier();
+        __syncthreads();  // the list is reused by the block's next row
     }
 }
 
@@ -1103,7 +1109,7 @@ void launch_exact_rows(const double* ru64, const double* qu64, int d, int64_t n_
     wf.out_idx = out_idx;
     wf.over_rows = er.over_rows;
     wf.over_count = er.over_count;
-    hipLaunchKernelGGL(wave_fallback_kernel, dim3(2048), dim3(64 * kFbWaves), 0, st, wf);
+    hipLaunchKernelGGL(row_fallback_kernel, dim3(2048), dim3(kFbThreads), 0, st, wf);
     SimpleArgs sa{};
     sa.ru64 = ru64;
     sa.qu64 = qu64;
