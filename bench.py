@@ -27,6 +27,7 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32_VALU_TFLOPS = 157.3
 PEAK_FP64_VALU_TFLOPS = 78.6
+PEAK_BF16_MFMA_TFLOPS = 2500.0   # dense bf16 matrix peak (no sparsity)
 PEAK_HBM_GBS = 8000.0
 
 
@@ -232,11 +233,45 @@ def c4_eval(num_nn):
     ks = nat.kernel_struct(params, mapping, len(kern.theta))
     ls = float(np.asarray(params["length_scale"]).reshape(-1)[0])
     F32 = hip_ops.PRECISION_FP32_REFINED
+    n, d = x.shape
+    nn = hip_ops.knn(x, x, ls, num_nn, ref_cut=0, precision=F32)   # warm-up
     torch.cuda.synchronize()
+    nat.timing_enable(True)
+    for k in (nat.TIMED_KNN_SCAN, nat.TIMED_KNN_RERANK):
+        nat.timing_read(k, reset=True)
     t0 = time.perf_counter()
     nn = hip_ops.knn(x, x, ls, num_nn, ref_cut=0, precision=F32)
     torch.cuda.synchronize()
     t_knn = time.perf_counter() - t0
+    screen_ms = nat.timing_read(nat.TIMED_KNN_SCAN)[0]
+    rerank_ms = nat.timing_read(nat.TIMED_KNN_RERANK)[0]
+    nat.timing_enable(False)
+    recomputed = hip_ops.knn_stats()["knn_rows_recomputed"]
+    # Matrix-core screen: the Vecchia self-search scores, for query tile t (32 rows),
+    # ref tiles 0..t (its rows' limits end inside tile t): n_t (n_t + 1) / 2 tiles of
+    # 32 x 32 pairs, each NK = ceil((3 d + 2) / 16) bf16 products of 32 x 32 x 16.
+    n_t = (n + 31) // 32
+    tiles = n_t * (n_t + 1) / 2.0
+    nk = -(-(3 * d + 2) // 16)
+    mfma_flop = tiles * nk * 2.0 * 32 * 32 * 16
+    ach = mfma_flop / (screen_ms * 1e-3) / 1e12
+    roofline_screen = {"kernel": "mfma_screen_kernel (bf16 hi/lo split products, FP32 accumulation)",
+                       "bound": "mfma", "achieved": ach, "peak": PEAK_BF16_MFMA_TFLOPS,
+                       "unit": "TFLOP/s", "frac": ach / PEAK_BF16_MFMA_TFLOPS,
+                       "avg_ms": screen_ms, "tiles": tiles, "products_per_tile": nk,
+                       "algorithmic_pairs": n * (n - 1) / 2.0,
+                       "fp32_equivalent_tflops": 3.0 * d * n * (n - 1) / 2.0 / (screen_ms * 1e-3) / 1e12}
+    t_valu = None
+    if num_nn == 32:
+        # the same search with the FP32 VALU screen (identical indices)
+        with hip_ops.knn_method(hip_ops.KNN_BRUTE_FORCE_VALU):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            nn_v = hip_ops.knn(x, x, ls, num_nn, ref_cut=0, precision=F32)
+            torch.cuda.synchronize()
+            t_valu = time.perf_counter() - t0
+        assert torch.equal(nn_v, nn), "screens disagree"
+        del nn_v
     hip_ops.vecchia_loglik(x, y, nn, ks, ds["alpha"], precision=F32, eval_gradient=True,
                            check_pd=False)
     torch.cuda.synchronize()
@@ -247,15 +282,18 @@ def c4_eval(num_nn):
                                check_pd=False)
     torch.cuda.synchronize()
     t_fused = (time.perf_counter() - t0) / reps
-    n, d = x.shape
     # minimum HBM bytes of the fused pass: neighbour indices, gathered coordinates
     # and targets of every neighbour and the point, per-point outputs
     min_bytes = n * (num_nn * 4 + (num_nn + 1) * (d + 1) * 8 + (1 + len(kern.theta)) * 8)
     out = {"workload": "uniform_8d: n=2,000,000 d=8 m=%d, White+C*RBF(0.3), FP32 factor + "
                        "FP64 refinement" % num_nn,
-           "knn_ms": t_knn * 1e3, "fused_loglik_grad_ms": t_fused * 1e3,
+           "knn_ms": t_knn * 1e3, "knn_screen_ms": screen_ms, "knn_rerank_ms": rerank_ms,
+           "knn_rows_recomputed": recomputed,
+           "knn_ms_valu_screen": t_valu * 1e3 if t_valu is not None else None,
+           "fused_loglik_grad_ms": t_fused * 1e3,
            "evals_per_s_knn_per_eval": 1.0 / (t_knn + t_fused),
-           "fused_min_hbm_gb_per_s": min_bytes / t_fused / 1e9}
+           "fused_min_hbm_gb_per_s": min_bytes / t_fused / 1e9,
+           "roofline_mfma_screen": roofline_screen}
     del x, y, nn
     torch.cuda.empty_cache()
     return out
