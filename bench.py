@@ -29,6 +29,13 @@ PEAK_FP32_VALU_TFLOPS = 157.3
 PEAK_FP64_VALU_TFLOPS = 78.6
 PEAK_BF16_MFMA_TFLOPS = 2500.0   # dense bf16 matrix peak (no sparsity)
 PEAK_HBM_GBS = 8000.0
+_T_START = time.perf_counter()
+
+
+def progress(msg):
+    """One line on stderr per finished phase (the run's own progress record)."""
+    print("bench: %s (%.1f s)" % (msg, time.perf_counter() - _T_START), file=sys.stderr,
+          flush=True)
 
 
 def parse():
@@ -459,8 +466,10 @@ def main():
         raise SystemExit("unknown extras: %s" % ", ".join(sorted(want - ALL_EXTRAS)))
     if not args.no_cpu:
         cpu = cpu_baseline(ds)  # before the GPU is touched (fork-based pool)
+        progress("cpu baseline done")
         if "sequential" in want:
             cpu_seq = cpu_sequential(ds)
+            progress("cpu sequential baseline done")
 
     import torch
     from nngpr_amd import _native as nat
@@ -546,6 +555,7 @@ def main():
         evaluate(HEAD)
     torch.cuda.synchronize()
     step_s, stages = stage_times(lambda: evaluate(HEAD), args.steps)
+    progress("headline timed region done: %.3f ms per step" % (step_s * 1e3))
     fused_avg = stages["fused_loglik_grad"] / max(stages["fused_loglik_grad_launches"], 1)
 
     # Roofline of the fused fill -> Cholesky -> solve -> loglik -> adjoint pass
@@ -643,18 +653,26 @@ def main():
             "roofline_knn_brute_force_scan": roofline_scan,
             "loglik": r64["ll"],
         })
+        progress("precision modes done")
 
     serving = None
     if "serving" in want:
         serving = serve_sweep(ds, (50, 100, 200, 500, 1000, 2000))
+        progress("serving sweep done")
     if "fit_loop" in want:
         extras["fit_loop_road_mixture"] = fit_loop(ds)
+        progress("fit loop done")
     if "fit_d18" in want:
         extras["correlated_gauss_d18"] = c3_fit()
+        progress("18-feature fit done")
     if "fit_predict_restarts" in want:
         extras["fit_predict_3_restarts"] = fit_predict_restarts(ds)
+        progress("fit + predict restarts done")
     if "uniform_8d" in want:
-        extras["uniform_8d_2m"] = [c4_eval(32), c4_eval(64)]
+        extras["uniform_8d_2m"] = [c4_eval(32)]
+        progress("2M-row 8-feature m=32 done")
+        extras["uniform_8d_2m"].append(c4_eval(64))
+        progress("2M-row 8-feature m=64 done")
     if "matern" in want:
         # the same evaluation with a Matern stationary part (same parameters)
         matern = {}
@@ -672,12 +690,14 @@ def main():
                                     max(st_m["fused_loglik_grad_launches"], 1),
                                     "loglik": float(ev_m()[0].item())}
         extras["matern"] = matern
+        progress("matern done")
     if "sequential" in want:
         seq = sequential_extras(x, y, xq, ks, ls, m, alpha, max(2, args.steps // 2))
         if cpu_seq is not None:
             seq["cpu_reference_loop"] = cpu_seq
             seq["speedup_vs_cpu_loop"] = seq["full_mean_std_queries_per_s"] / cpu_seq["queries_per_s"]
         extras["sequential_conditioning"] = seq
+        progress("sequential conditioning done")
 
     line = {
         "metric": "loglik+grad evals/s (reference default precision 'only-nn': FP64 factor, "

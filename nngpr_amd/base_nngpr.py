@@ -296,8 +296,10 @@ class Nngpr(GaussianProcessRegressor):
             return nn
         lists.clear()
         K = k + self._KNN_REUSE_MARGIN
+        # (reuse needs a search for K = k + margin candidates: only where that search
+        # is as fast as the k-list search it replaces)
         if scale is None or not getattr(self, "_knn_reuse", True) or \
-                K > hip_ops.RESCALE_MAX_K or k > nat.NNGP_MAX_K:
+                K > min(hip_ops.RESCALE_MAX_K, hip_ops.KNN_FAST_MAX_K) or k > nat.NNGP_MAX_K:
             nn = _knn_lists(x_dev, x_dev, scale, k, ref_cut=0, precision=precision)
         else:
             base = knn_cache.get("rescaler")
@@ -308,6 +310,15 @@ class Nngpr(GaussianProcessRegressor):
             if base is not None and knn_cache.pop("check", False) and \
                     base.rows_recomputed() > self._KNN_REBASE_FRACTION * n:
                 base = None
+            # A row stays certified when max_f(r_f)^2 times its m-th old key is below
+            # min_f(r_f)^2 times its K-th (r_f = old / new length scale); with keys
+            # growing like rank^(2/d) that fails for most rows once
+            # max(r) / min(r) > (K / k)^(1/d), and every failing row is then searched
+            # alone over all its refs: a new search is far cheaper.
+            if base is not None and scale.size > 1:
+                ratio = base.scale / scale
+                if ratio.max() > ratio.min() * (K / k) ** (1.0 / x_dev.shape[1]):
+                    base = None
             if base is None:
                 base = hip_ops.KnnRescaler(x_dev, x_dev, scale, k, min(K, max(n - 1, k)),
                                            ref_cut=0, precision=precision)

@@ -106,6 +106,9 @@ def knn(ref, query, scale, k, ref_cut=None, precision=PRECISION_FP32_REFINED, ou
 
 
 RESCALE_MAX_K = 72
+# Largest k of the parallel searches (uniform grid, brute-force scans); larger
+# lists are searched one row per thread, far slower.
+KNN_FAST_MAX_K = 64
 
 
 class KnnRescaler:

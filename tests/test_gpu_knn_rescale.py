@@ -118,3 +118,36 @@ def test_fit_reuse_follows_the_same_path(ard):
         fits.append(est)
     np.testing.assert_array_equal(fits[0].kernel_.theta, fits[1].kernel_.theta)
     assert fits[0].log_marginal_likelihood_value_ == fits[1].log_marginal_likelihood_value_
+
+
+def test_fit_reuse_policy():
+    """When a search is reused and when a new one runs: lists of k + 8 above the
+    parallel searches' limit are never reused (every evaluation searches k
+    lists), an anisotropic length-scale change beyond (K / k)^(1/d) starts a new
+    search, a smaller one rescales the stored candidates; the log-likelihood is
+    the same as with a search at every evaluation in every case."""
+    ds = datasets.road_mixture(n_train=20_000, n_test=10)
+    x, y = ds["x_train"], ds["y_train"]
+    kern = bk.WhiteKernel(0.01) + bk.ConstantKernel(1.0) * bk.RBF(np.array([0.1, 0.1, 0.1]))
+    ref = TorchNngpr("cuda:0", kernel=kern, num_nn=16, alpha=1e-10, optimizer=None)
+    ref._knn_reuse = False
+    ref.fit(x, y)
+    est = TorchNngpr("cuda:0", kernel=kern, num_nn=16, alpha=1e-10, optimizer=None)
+    est.fit(x, y)
+    cache = est._dev_cache[3]
+    base = cache.get("rescaler")
+    assert base is not None
+    th0 = est.kernel_.theta.copy()
+    th_small = th0.copy()
+    th_small[2:5] += np.log([1.0, 1.01, 1.02])   # spread 1.02 < (24 / 16)^(1/3)
+    th_big = th0.copy()
+    th_big[2:5] += np.log([1.0, 1.3, 0.8])       # spread 1.63
+    for th, same_base in ((th_small, True), (th_big, False)):
+        ll = est.log_marginal_likelihood(th)
+        assert (cache.get("rescaler") is base) == same_base
+        base = cache.get("rescaler")
+        assert ll == ref.log_marginal_likelihood(th)
+    est64 = TorchNngpr("cuda:0", kernel=kern, num_nn=64, alpha=1e-10, optimizer=None)
+    est64.fit(x, y)
+    est64.log_marginal_likelihood(th_small)
+    assert est64._dev_cache[3].get("rescaler") is None
