@@ -210,6 +210,10 @@ class _Slot:
             qh[off:off + rows] = req.x[r0:r0 + rows]
             off += rows
         b = self.bucket(npts)
+        # padding rows repeat the batch's first query: rows left from earlier batches
+        # or zeros could lie far outside the training set, and a far query walks
+        # many grid rings (the padded batch's kNN would wait for it)
+        qh[npts:b] = qh[0]
         with torch.cuda.stream(self.stream):
             self.q_dev[:b].copy_(self.q_host[:b], non_blocking=True)
             if self.use_graphs:
