@@ -121,14 +121,16 @@ SIGNATURES = [
 TIMED_KNN_SCAN, TIMED_KNN_RERANK, TIMED_LOGLIK, TIMED_PREDICT = 0, 1, 2, 3
 TIMED_KNN_GRID, TIMED_KNN_GRID_BUILD = 4, 5
 
-KNN_AUTO, KNN_BRUTE_FORCE, KNN_BRUTE_FORCE_VALU = 0, 1, 2
+KNN_AUTO, KNN_BRUTE_FORCE, KNN_BRUTE_FORCE_VALU, KNN_GRID_LANES, KNN_GRID_WAVES = 0, 1, 2, 3, 4
 
 
 def knn_set_method(method):
-    """Process-wide kNN search method (KNN_AUTO: grid search for d <= 3, k <= 64;
+    """Process-wide kNN search method (KNN_AUTO: grid search for d <= 3, k <= 64,
+    one query per wave up to 65,536 queries and one per lane above;
     KNN_BRUTE_FORCE: always the scans; both screen d >= 4 on the matrix cores in
-    FP32_REFINED; KNN_BRUTE_FORCE_VALU: the scans with the FP32 VALU screen).
-    Every method returns identical indices."""
+    FP32_REFINED; KNN_BRUTE_FORCE_VALU: the scans with the FP32 VALU screen;
+    KNN_GRID_LANES / KNN_GRID_WAVES: AUTO with the one grid kernel for every
+    query count).  Every method returns identical indices."""
     check(lib().nngp_knn_set_method(int(method)), "nngp_knn_set_method")
 
 

@@ -1131,7 +1131,8 @@ int knn_method() { return g_knn_method.load(); }
 
 int knn_set_method(int method) {
     if (method != NNGP_KNN_AUTO && method != NNGP_KNN_BRUTE_FORCE &&
-        method != NNGP_KNN_BRUTE_FORCE_VALU)
+        method != NNGP_KNN_BRUTE_FORCE_VALU && method != NNGP_KNN_GRID_LANES &&
+        method != NNGP_KNN_GRID_WAVES)
         return NNGP_ERR_INVALID_ARGUMENT;
     g_knn_method.store(method);
     return NNGP_OK;
@@ -1141,7 +1142,8 @@ int knn_set_method(int method) {
 // are prepared).
 int refs_layout(int d, int precision) {
     const int m = knn_method();
-    if (d <= 3 && m == NNGP_KNN_AUTO) return kRefGrid;
+    if (d <= 3 && (m == NNGP_KNN_AUTO || m == NNGP_KNN_GRID_LANES || m == NNGP_KNN_GRID_WAVES))
+        return kRefGrid;
     if (precision == NNGP_PRECISION_FP32_REFINED && mfma_screen_supported(d) &&
         m != NNGP_KNN_BRUTE_FORCE_VALU)
         return kRefMfma;

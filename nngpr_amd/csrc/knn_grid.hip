@@ -26,6 +26,14 @@
 // group of 64 queries grown by the neighbour radius holds ~20x the pairs the
 // lanes need, at about the same VALU cost per pair.)
 //
+// Searches of at most kWaveSearchMaxQueries queries run one query per wave
+// instead: the same rings, stop test and run clipping, with the wave's lanes
+// keying 64 refs of the ring's runs at a time and the 64 smallest pairs kept
+// sorted across the lanes.  A lane-per-query wave's latency (its lanes' ring walks
+// one after the other, ~0.5 ms on the road-mixture data) would otherwise bound
+// every search below ~10^5 queries; on the full 434,874-row Vecchia search a wave
+// per query costs 2.7x the lane kernel.
+//
 // Vecchia cut (row q searches refs [0, q + cut)): rows are split into bands of
 // doubling limit [L, 2L).  A band's grid holds refs [0, 2L) only, so at least half
 // of the refs a band streams are admissible for each of its rows.  Rows with
@@ -52,6 +60,13 @@ constexpr int kCoopSlots = 16;     // LDS append slots per lane
 // take the instantiation that walks only the boundary rows of rings wider than the
 // x extent.
 constexpr int kNarrowCells = 16;
+// Searches of at most this many queries run one query per wave.  A lane-per-query
+// wave walks its 64 queries' rings one after the other: ~0.5-0.7 ms on the
+// road-mixture data for any query search of 64 to 87,000 queries against its
+// 434,874 refs.  A wave per query keys 64 refs per step and ends in tens of
+// microseconds: 0.17 ms (64 queries, grid build included), 0.40 ms (43,487),
+// 0.63 ms (86,974); on the full Vecchia self-search it costs 2.7x the lane kernel.
+constexpr int64_t kWaveSearchMaxQueries = 65536;
 
 __device__ __forceinline__ bool point_finite(const double* u, int d) {
     bool fin = true;
@@ -915,6 +930,296 @@ void lane_knn_kernel(SearchArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Wave-per-query search
+// ---------------------------------------------------------------------------
+// Value of lane `src` (wave-uniform) of a double / int.
+__device__ __forceinline__ double lane_d(double v, int src) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), src);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Candidates of one chunk merged one by one at most this many, else by a sort.
+constexpr int kWaveInsertMax = 6;
+
+// One query per wave, every control decision wave-uniform.  The wave keeps the 64
+// smallest admitted (key, index) pairs sorted across its lanes (lane t: the t-th),
+// so the admission threshold is the pair in lane k - 1.  Ring R of cells around
+// the query's cell is enumerated one row (y, z) per lane: a row's whole x run when
+// it is a boundary row of the ring, else its two end cells, each run clipped to
+// the ball of the current admission radius.  The refs of the rows' runs are then
+// examined 64 at a time (lane l takes position l of the rows' concatenation, found
+// by a binary search over the exclusive prefix of the run lengths): the FP64 key,
+// the Vecchia limit and the threshold.  A few admitted refs are inserted into the
+// list one by one; more are sorted by a bitonic network and merged (elementwise
+// minimum against the reversed chunk, then half-cleaners).  After ring R the
+// query is done when its k-th key is below the squared distance to the nearest
+// face of the visited block with cells beyond it (minus tol), or when the block
+// covers the grid.  Rings whose interior rows have both end cells off the grid
+// (wider than the grid in x) enumerate their boundary rows only.
+template <int D>
+__global__ __launch_bounds__(256) void wave_knn_kernel(SearchArgs a) {
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int64_t gq = (int64_t)blockIdx.x * 4 + w;  // query slot over all bands
+    if (gq >= a.n_waves) return;
+    int bj = 0;
+    for (int t = 1; t < a.n_bands; ++t)
+        if (a.band[t].wave_off <= gq) bj = t;
+    const BandDesc& bd = a.band[bj];
+    const GridPt qp = bd.qpts[gq - bd.wave_off];
+    double qu[3];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) qu[f] = qp.u[f];
+    const int64_t q = qp.idx;
+    int64_t lim = a.n_ref;
+    if (a.has_cut) {
+        lim = q + a.cut;
+        lim = lim < 0 ? 0 : (lim > a.n_ref ? a.n_ref : lim);
+    }
+    bool fin = true;
+#pragma unroll
+    for (int f = 0; f < D; ++f) fin = fin && __builtin_isfinite(qu[f]);
+    const int k = a.k;
+    // +inf through a register: held as a constant in an SGPR pair it was emitted as
+    // s_mov_b64 with a 64-bit literal, which this toolchain encoded as 0 for gfx950
+    double bk = __builtin_inf();
+    asm volatile("" : "+v"(bk));
+    int bi = -1;
+    uint32_t examined = 0;
+    if (fin && lim > 0) {
+        const GridParams& P = *bd.gp;
+        int g[3], c[3];
+        double lo[3];
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            g[f] = P.g[f];
+            lo[f] = P.lo[f];
+            c[f] = f < D ? cell_coord(qu[f], lo[f], P.inv_h, g[f]) : 0;
+        }
+        const double h = P.h, inv_h = P.inv_h, tol = P.tol;
+        const double qxc = (qu[0] - lo[0]) * inv_h;
+        const int32_t* cs = bd.cell_start;
+        const GridPt* rpts = bd.rpts;
+        const int32_t lim32 = (int32_t)lim;
+        double tk, rr = bk;  // admission threshold pair (the k-th); squared clip radius
+        int ti;
+        auto set_threshold = [&]() {
+            tk = lane_d(bk, k - 1);
+            ti = __builtin_amdgcn_readlane(bi, k - 1);
+            if (tk < __builtin_inf()) {
+                const double r = sqrt(tk) + tol;
+                rr = r * r;
+            }
+        };
+        set_threshold();
+#pragma unroll 1
+        for (int R = 0;; ++R) {
+            const int y0 = max(c[1] - R, 0), y1 = min(c[1] + R, g[1] - 1);
+            const int z0 = max(c[2] - R, 0), z1 = min(c[2] + R, g[2] - 1);
+            const int ny = y1 - y0 + 1, nz = z1 - z0 + 1;
+            const bool edge = R > 0 && c[0] - R < 0 && c[0] + R > g[0] - 1;
+            const int nbot = c[2] - R >= 0 ? ny : 0;
+            const int ntop = (R > 0 && c[2] + R <= g[2] - 1) ? ny : 0;
+            const int zm0 = max(c[2] - R + 1, 0), zm1 = min(c[2] + R - 1, g[2] - 1);
+            const int nzm = max(zm1 - zm0 + 1, 0);
+            const bool ylo = c[1] - R >= 0, yhi = R > 0 && c[1] + R <= g[1] - 1;
+            const int nside = (int)ylo + (int)yhi;
+            const int nrows = edge ? nbot + ntop + nzm * nside : ny * nz;
+#pragma unroll 1
+            for (int rb = 0; rb < nrows; rb += 64) {
+                const int r = rb + lane;
+                int y = 0, z = 0;
+                if (!edge) {
+                    y = y0 + r % ny;
+                    z = z0 + r / ny;
+                } else {
+                    int rr2 = r;
+                    if (rr2 < nbot) {
+                        y = y0 + rr2;
+                        z = c[2] - R;
+                    } else if ((rr2 -= nbot) < ntop) {
+                        y = y0 + rr2;
+                        z = c[2] + R;
+                    } else {
+                        rr2 -= ntop;
+                        z = zm0 + (nside == 2 ? (rr2 >> 1) : rr2);
+                        y = (ylo && !(nside == 2 && (rr2 & 1))) ? c[1] - R : c[1] + R;
+                    }
+                }
+                int a0 = 1, b0 = 0, a1 = 1, b1 = 0;
+                if (r < nrows) {
+                    const bool full = R == 0 || z == c[2] - R || z == c[2] + R ||
+                                      y == c[1] - R || y == c[1] + R;
+                    if (full) {
+                        a0 = max(c[0] - R, 0);
+                        b0 = min(c[0] + R, g[0] - 1);
+                    } else {
+                        if (c[0] - R >= 0) a0 = b0 = c[0] - R;
+                        if (c[0] + R <= g[0] - 1) a1 = b1 = c[0] + R;
+                    }
+                    if (rr < __builtin_inf()) {
+                        // clip to the ball (cell units, widened as in lane_knn_kernel)
+                        double s2 = 0.0;
+                        if (D >= 2) {
+                            const double yl = lo[1] + (double)y * h;
+                            const double dy = fmax(fmax(yl - qu[1], qu[1] - (yl + h)), 0.0);
+                            s2 = dy * dy;
+                        }
+                        if (D >= 3) {
+                            const double zl = lo[2] + (double)z * h;
+                            const double dz = fmax(fmax(zl - qu[2], qu[2] - (zl + h)), 0.0);
+                            s2 = fma(dz, dz, s2);
+                        }
+                        const double rem = rr - s2;
+                        int xl = 0, xh = -1;
+                        if (rem >= 0.0) {
+                            const double rc =
+                                (double)(__builtin_sqrtf((float)(rem * inv_h * inv_h)) *
+                                             1.000001f +
+                                         1.0e-6f);
+                            const double gx = (double)g[0];
+                            xl = (int)fmin(fmax(floor(qxc - rc), -1.0), gx);
+                            xh = (int)fmin(fmax(floor(qxc + rc), -1.0), gx);
+                        }
+                        a0 = max(a0, xl);
+                        b0 = min(b0, xh);
+                        a1 = max(a1, xl);
+                        b1 = min(b1, xh);
+                    }
+                }
+                const int64_t rowb = (int64_t)g[0] * (y + (int64_t)g[1] * z);
+                int p0 = 0, n0 = 0, p1 = 0, n1 = 0;
+                if (a0 <= b0) {
+                    p0 = cs[rowb + a0];
+                    n0 = cs[rowb + b0 + 1] - p0;
+                }
+                if (a1 <= b1) {
+                    p1 = cs[rowb + a1];
+                    n1 = cs[rowb + b1 + 1] - p1;
+                }
+                // exclusive prefix of the lanes' ref counts
+                const int cntl = n0 + n1;
+                int incl = cntl;
+#pragma unroll
+                for (int off = 1; off < 64; off <<= 1) {
+                    const int o = __shfl_up(incl, off, 64);
+                    if (lane >= off) incl += o;
+                }
+                const int ex = incl - cntl;
+                const int T = __builtin_amdgcn_readlane(incl, 63);
+                examined += (uint32_t)T;
+#pragma unroll 1
+                for (int cb = 0; cb < T; cb += 64) {
+                    const int Pn = cb + lane;
+                    // the lane whose run holds position Pn: the last with ex <= Pn
+                    int src = 0, es = 0;
+#pragma unroll
+                    for (int s = 32; s > 0; s >>= 1) {
+                        const int e = __shfl(ex, src + s, 64);
+                        if (e <= Pn) {
+                            src += s;
+                            es = e;
+                        }
+                    }
+                    const int o = Pn - es;
+                    const int sp0 = __shfl(p0, src, 64);
+                    const int sn0 = __shfl(n0, src, 64);
+                    const int sp1 = __shfl(p1, src, 64);
+                    const bool pv = Pn < T;
+                    const int pos = pv ? (o < sn0 ? sp0 + o : sp1 + (o - sn0)) : 0;
+                    const GridPt* rq = rpts + pos;
+                    const u32x4 w01 = *reinterpret_cast<const u32x4*>(rq);
+                    const u32x3 w2 = *reinterpret_cast<const u32x3*>(&rq->u[2]);
+                    double ru[3];
+                    ru[0] = __hiloint2double((int)w01.y, (int)w01.x);
+                    ru[1] = __hiloint2double((int)w01.w, (int)w01.z);
+                    ru[2] = __hiloint2double((int)w2.y, (int)w2.x);
+                    const int ri = (int)w2.z;
+                    const double t0 = ru[0] - qu[0];
+                    double key = t0 * t0;
+#pragma unroll
+                    for (int f = 1; f < D; ++f) {
+                        const double tt = ru[f] - qu[f];
+                        key = fma(tt, tt, key);
+                    }
+                    const bool adm = pv & (ri < lim32) & pair_less(key, ri, tk, ti);
+                    uint64_t bal = __builtin_amdgcn_ballot_w64(adm);
+                    if (!bal) continue;
+                    if (__builtin_popcountll(bal) <= kWaveInsertMax) {
+                        do {
+                            const int s = (int)__builtin_ctzll(bal);
+                            bal &= bal - 1;
+                            const double ck = lane_d(key, s);
+                            const int ci = __builtin_amdgcn_readlane(ri, s);
+                            if (!pair_less(ck, ci, tk, ti)) continue;
+                            const uint64_t m = __builtin_amdgcn_ballot_w64(pair_less(ck, ci, bk, bi));
+                            const int at = (int)__builtin_ctzll(m);
+                            const double pk = __shfl_up(bk, 1, 64);
+                            const int pi = __shfl_up(bi, 1, 64);
+                            if (lane > at) {
+                                bk = pk;
+                                bi = pi;
+                            } else if (lane == at) {
+                                bk = ck;
+                                bi = ci;
+                            }
+                            set_threshold();
+                        } while (bal);
+                    } else {
+                        double sk = adm ? key : __builtin_inf();
+                        int si = adm ? ri : -1;
+                        for (int size = 2; size <= 64; size <<= 1)
+                            for (int stride = size >> 1; stride > 0; stride >>= 1)
+                                wave_pair_step(sk, si, lane, size, stride);
+                        const double rk = __shfl(sk, 63 - lane, 64);
+                        const int rix = __shfl(si, 63 - lane, 64);
+                        if (pair_less(rk, rix, bk, bi)) {
+                            bk = rk;
+                            bi = rix;
+                        }
+                        for (int stride = 32; stride > 0; stride >>= 1)
+                            wave_pair_step(bk, bi, lane, 64, stride);
+                        set_threshold();
+                    }
+                }
+            }
+            // stop test: the nearest face of the visited block with cells beyond it
+            double bdist = __builtin_inf();
+#pragma unroll
+            for (int f = 0; f < D; ++f) {
+                if (c[f] - R >= 1) bdist = fmin(bdist, qu[f] - (lo[f] + (double)(c[f] - R) * h));
+                if (c[f] + R + 1 <= g[f] - 1)
+                    bdist = fmin(bdist, (lo[f] + (double)(c[f] + R + 1) * h) - qu[f]);
+            }
+            if (bdist == __builtin_inf()) break;  // every cell visited
+            const double bb = bdist - tol;
+            if (bb > 0.0 && tk < bb * bb) break;
+        }
+    }
+    if (lane < k) a.out_idx[q * k + lane] = bi;
+    if (a.pair_count && lane == 0 && examined) atomicAdd(a.pair_count, (unsigned long long)examined);
+}
+
+// One launch over every band's queries, one wave each (band table: wave_off =
+// first query slot of the band).
+void launch_wave_search(SearchArgs& c, int d, hipStream_t st) {
+    int64_t nq = 0;
+    for (int t = 0; t < c.n_bands; ++t) {
+        c.band[t].wave_off = nq;
+        nq += c.band[t].n_q;
+    }
+    if (nq == 0) return;
+    c.n_waves = nq;
+    const unsigned blocks = (unsigned)((nq + 3) / 4);
+    if (d == 1) hipLaunchKernelGGL(wave_knn_kernel<1>, dim3(blocks), dim3(256), 0, st, c);
+    else if (d == 2) hipLaunchKernelGGL(wave_knn_kernel<2>, dim3(blocks), dim3(256), 0, st, c);
+    else hipLaunchKernelGGL(wave_knn_kernel<3>, dim3(blocks), dim3(256), 0, st, c);
+}
+
+
 // Estimated cost of each search wave: the largest number of refs in the 3 x 3 x 3
 // cells around one of its queries.
 __global__ __launch_bounds__(64 * kCoopWaves) void wave_cost_kernel(SearchArgs a,
@@ -1195,7 +1500,14 @@ int grid_search(const GridRefMem& G, const double* ru64, int64_t n_ref, int d,
     timing_end(TK_KNN_GRID_BUILD, st);
     sa.pair_count = timing_work_counter(TK_KNN_GRID);
     timing_begin(TK_KNN_GRID, st);
-    launch_search(sa, d, M, st);
+    const int meth = knn_method();
+    int64_t n_searched = 0;
+    for (int t = 0; t < sa.n_bands; ++t) n_searched += sa.band[t].n_q;
+    if (meth == NNGP_KNN_GRID_WAVES ||
+        (meth != NNGP_KNN_GRID_LANES && n_searched <= kWaveSearchMaxQueries))
+        launch_wave_search(sa, d, st);
+    else
+        launch_search(sa, d, M, st);
     timing_end(TK_KNN_GRID, st);
     NNGP_HIP_CHECK(hipGetLastError());
     return NNGP_OK;

@@ -16,6 +16,8 @@ PRECISION_FP32_REFINED = nat.PRECISION_FP32_REFINED
 KNN_AUTO = nat.KNN_AUTO
 KNN_BRUTE_FORCE = nat.KNN_BRUTE_FORCE
 KNN_BRUTE_FORCE_VALU = nat.KNN_BRUTE_FORCE_VALU
+KNN_GRID_LANES = nat.KNN_GRID_LANES
+KNN_GRID_WAVES = nat.KNN_GRID_WAVES
 
 _workspaces = {}
 

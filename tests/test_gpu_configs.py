@@ -26,7 +26,7 @@ from oracle import vecchia_oracle as vo
 
 pytestmark = pytest.mark.gpu
 DEV = "cuda:0"
-METHODS = [hip_ops.KNN_AUTO, hip_ops.KNN_BRUTE_FORCE]
+METHODS = [hip_ops.KNN_AUTO, hip_ops.KNN_BRUTE_FORCE, hip_ops.KNN_GRID_WAVES]
 
 
 def t(a, dtype=torch.float64):

@@ -1,7 +1,7 @@
 """HIP kNN vs the FP64 oracle: neighbour indices must be bit-identical.
 
-Covers every search form (uniform-grid search for d <= 3; brute-force exact FP64
-scan; FP32 screen + FP64 re-rank with certificate), Vecchia cuts like the
+Covers every search form (uniform-grid search for d <= 3, one query per lane
+and one query per wave; brute-force exact FP64 scan; FP32 screen + FP64 re-rank with certificate), Vecchia cuts like the
 reference's tests (None, 0, 2, -1), isotropic / ARD / raw coordinates, ties and
 duplicates, non-finite coordinates, clustered and degenerate point sets, queries
 far outside the refs, empty inputs, k > 64 and d > 8 (per-row path), the stored
@@ -25,10 +25,13 @@ def t(a):
     return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(DEV)
 
 
-METHODS = [hip_ops.KNN_AUTO, hip_ops.KNN_BRUTE_FORCE, hip_ops.KNN_BRUTE_FORCE_VALU]
+# AUTO runs the grid search one query per wave at these sizes; GRID_LANES forces
+# the lane-per-query grid kernel, GRID_WAVES the wave-per-query one at every size
+METHODS = [hip_ops.KNN_AUTO, hip_ops.KNN_BRUTE_FORCE, hip_ops.KNN_BRUTE_FORCE_VALU,
+           hip_ops.KNN_GRID_LANES, hip_ops.KNN_GRID_WAVES]
 
 
-@pytest.fixture(params=METHODS, ids=["auto", "brute", "valu"])
+@pytest.fixture(params=METHODS, ids=["auto", "brute", "valu", "lanes", "waves"])
 def method(request):
     with hip_ops.knn_method(request.param):
         yield request.param

@@ -95,8 +95,9 @@ def test_knn_method_setting():
     assert nat.knn_get_method() == nat.KNN_AUTO
     nat.knn_set_method(nat.KNN_BRUTE_FORCE)
     assert nat.knn_get_method() == nat.KNN_BRUTE_FORCE
-    nat.knn_set_method(nat.KNN_AUTO)
-    assert nat.knn_get_method() == nat.KNN_AUTO
+    for m in (nat.KNN_GRID_LANES, nat.KNN_GRID_WAVES, nat.KNN_AUTO):
+        nat.knn_set_method(m)
+        assert nat.knn_get_method() == m
     with pytest.raises(ValueError):
         nat.knn_set_method(7)
     assert nat.knn_get_method() == nat.KNN_AUTO
