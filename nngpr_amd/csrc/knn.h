@@ -24,6 +24,12 @@ struct GridParams {
     double h;
     double inv_h;
     double tol;       // slack subtracted from every distance lower bound
+    // A finite point with a coordinate outside [in_lo, in_hi] is kept out of the
+    // cells: the "outside" bucket of refs, keyed by every search, or the far
+    // bucket of queries.  The box is the grid's own extent when a few far points
+    // would otherwise stretch the grid (see setup_grid), else unbounded.
+    double in_lo[3];
+    double in_hi[3];
     int g[3];
     int n_cells;
     int t[3];
@@ -31,9 +37,9 @@ struct GridParams {
     int n_qb;
 };
 
-// A grid over a set of refs and the refs sorted by cell.  Bucket `cap` collects
-// points with a non-finite coordinate (never neighbours); start has cap + 2
-// entries.
+// A grid over a set of refs and the refs sorted by cell.  Bucket `cap` collects the
+// finite points outside the grid's box, bucket `cap + 1` the points with a
+// non-finite coordinate (never neighbours); start has cap + 3 entries.
 struct GridMem {
     GridParams* params;
     int32_t* cnt;
@@ -54,7 +60,8 @@ struct GridRefMem {
     GridMem full;
 };
 
-// Query buckets of one band (tile order over the band's grid).
+// Query buckets of one band (tile order over the band's grid; bucket capq holds the
+// finite queries outside the grid's box, capq + 1 the non-finite ones).
 struct GridQueryBins {
     int32_t* cnt;
     int32_t* start;

@@ -10,7 +10,15 @@
 // Grid build: finite bounding box -> cell size h for about kPointsPerCell refs per
 // cell -> counting sort of the refs by cell (count, exclusive scan, atomic
 // scatter; the order inside a cell is arbitrary and does not matter because the
-// top-k is ordered by (key, index)).
+// top-k is ordered by (key, index)).  A few far refs would stretch the box and put
+// most refs into a few cells: when at most kMaxOutside refs fall outside an inner
+// box sampled from the refs (second smallest / largest of 128 sampled refs per
+// feature, widened by half their distance) and that box holds under 1/8 of the
+// volume, the grid spans the refs inside it and the others go to an "outside"
+// bucket that every search keys before its first ring.  The refs in the cells then
+// still lie inside their cells, so every distance bound of the search holds for any
+// query, inside or outside the box; queries outside it (which would walk many
+// rings) are searched one per wave.
 //
 // Search, one lane per query, 64 queries per wave: the queries are sorted by tiles
 // of cells, so a wave's lanes search neighbouring space.  Each lane walks rings of
@@ -75,6 +83,16 @@ __device__ __forceinline__ bool point_finite(const double* u, int d) {
 }
 
 constexpr int kBBoxBlocks = 256;  // blocks of the bounding-box pass (one partial each)
+// Box memory (long long words): [0, 6) the bounding box of the finite refs as
+// order-preserving keys, [8, 14) the candidate inner box, 14 the count of finite
+// refs outside it, [16, 22) the bounding box of the refs inside it, block partials
+// of the bounding-box pass from 24.
+constexpr int kBoxInner = 8;
+constexpr int kBoxOutCount = 14;
+constexpr int kBoxTight = 16;
+constexpr int kBoxPartials = 24;
+constexpr int kBoxSample = 128;   // refs sampled for the inner box
+constexpr int kMaxOutside = 256;  // most refs a narrowed grid keeps out of its cells
 
 __device__ __forceinline__ void bbox_wave_reduce(long long (&lo)[3], long long (&hi)[3]) {
 #pragma unroll
@@ -168,6 +186,136 @@ __global__ __launch_bounds__(256) void grid_bbox_final_kernel(const long long* p
         }
 }
 
+// Candidate inner box: per feature, the second smallest and second largest of
+// kBoxSample finite refs sampled at even strides of [0, n) (rank by (value, sample)),
+// widened on each side by half their distance; unbounded with fewer than 64 finite
+// samples.  Clears the outside count and the inside box.  One block; n > 0.
+__global__ __launch_bounds__(kBoxSample) void grid_box_sample_kernel(const double* u64, int64_t n,
+                                                                     int d, long long* bb) {
+    __shared__ double v[3][kBoxSample];
+    __shared__ double qv[3][2];
+    const int t = threadIdx.x;
+    int64_t i = (int64_t)(((double)t + 0.5) * (double)n / (double)kBoxSample);
+    i = i < n ? i : n - 1;
+    const double* u = u64 + i * d;
+    const bool fin = point_finite(u, d);
+    double x[3];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        x[f] = (f < d && fin) ? u[f] : __builtin_inf();
+        v[f][t] = x[f];
+    }
+    const int nf = __syncthreads_count(fin);
+    if (t == 0) {
+        bb[kBoxOutCount] = 0;
+        for (int f = 0; f < 3; ++f) {
+            bb[kBoxTight + 2 * f] = 0x7fffffffffffffffLL;
+            bb[kBoxTight + 2 * f + 1] = (long long)0x8000000000000000ULL;
+        }
+    }
+    // the three features' ranks together, 8 samples per step (independent loads)
+    int rank[3] = {0, 0, 0};
+#pragma unroll 8
+    for (int j = 0; j < kBoxSample; ++j) {
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            const double y = v[f][j];
+            rank[f] += (int)((y < x[f]) | ((y == x[f]) & (j < t)));
+        }
+    }
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        if (fin && rank[f] == 1) qv[f][0] = x[f];
+        if (fin && rank[f] == nf - 2) qv[f][1] = x[f];
+    }
+    __syncthreads();
+    if (t == 0) {
+        for (int f = 0; f < 3; ++f) {
+            double lo = -__builtin_inf(), hi = __builtin_inf();
+            if (f < d && nf >= 64) {
+                const double pad = 0.5 * (qv[f][1] - qv[f][0]);
+                lo = qv[f][0] - pad;
+                hi = qv[f][1] + pad;
+            }
+            bb[kBoxInner + 2 * f] = dbl_key(lo);
+            bb[kBoxInner + 2 * f + 1] = dbl_key(hi);
+        }
+    }
+}
+
+// Volume of the candidate inner box over the bounding box's (features of zero
+// extent left out); the inner box clipped to the bounding box in lo / ext.
+__device__ __forceinline__ double inner_box_ratio(const long long* bb, int d, double (&lo)[3],
+                                                  double (&ext)[3]) {
+    double ratio = 1.0;
+    for (int f = 0; f < d; ++f) {
+        const double l = key_dbl(bb[2 * f]), h = key_dbl(bb[2 * f + 1]);
+        const double il = fmax(key_dbl(bb[kBoxInner + 2 * f]), l);
+        const double ih = fmin(key_dbl(bb[kBoxInner + 2 * f + 1]), h);
+        lo[f] = il;
+        ext[f] = fmax(ih - il, 0.0);
+        const double e = h - l;
+        if (e > 0.0) ratio *= ext[f] / e;
+    }
+    return ratio;
+}
+
+// A narrowed grid must shrink the box's volume at least this much.
+constexpr double kNarrowVolume = 0.125;
+
+// Finite refs outside the candidate inner box, and the bounding box of those inside
+// it (only when that box is small enough to be used): block partials added to
+// bb[kBoxOutCount], bb[kBoxTight ...].
+__global__ __launch_bounds__(256) void grid_outside_count_kernel(const double* u64, int64_t n,
+                                                                 int d, long long* bb) {
+    double lo[3], ext[3];
+    if (!(inner_box_ratio(bb, d, lo, ext) < kNarrowVolume)) return;
+    double il[3], ih[3];
+    long long tl[3], th[3];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        il[f] = key_dbl(bb[kBoxInner + 2 * f]);
+        ih[f] = key_dbl(bb[kBoxInner + 2 * f + 1]);
+        tl[f] = 0x7fffffffffffffffLL;
+        th[f] = (long long)0x8000000000000000ULL;
+    }
+    unsigned long long c = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double* u = u64 + i * d;
+        if (!point_finite(u, d)) continue;
+        bool out = false;
+        for (int f = 0; f < d; ++f) out = out || u[f] < il[f] || u[f] > ih[f];
+        if (out) {
+            ++c;
+            continue;
+        }
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            if (f < d) {
+                const long long k = dbl_key(u[f]);
+                tl[f] = k < tl[f] ? k : tl[f];
+                th[f] = k > th[f] ? k : th[f];
+            }
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+    __shared__ unsigned long long wc[4];
+    if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = c;
+    bbox_wave_reduce(tl, th);
+    bbox_block_reduce(tl, th);  // (its barriers also publish wc)
+    if (threadIdx.x == 0) {
+        const unsigned long long tot = wc[0] + wc[1] + wc[2] + wc[3];
+        if (tot) atomicAdd(reinterpret_cast<unsigned long long*>(bb + kBoxOutCount), tot);
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            atomicMin(bb + kBoxTight + 2 * f, tl[f]);
+            atomicMax(bb + kBoxTight + 2 * f + 1, th[f]);
+        }
+    }
+}
+
 // Cell size and grid dimensions from the bounding box, for a grid over n refs.  The cell size targets n / kPointsPerCell cells over the box; it
 // grows until the grid fits in `cap` cells.  Query tiles are 4 cells wide along
 // features with at least 8 cells; the tiled bucket count must fit in capq.
@@ -185,6 +333,31 @@ __device__ void setup_grid(const long long* bb, int d, int64_t n, int cap, int c
         for (int f = 0; f < 3; ++f) {
             lo[f] = 0.0;
             ext[f] = 0.0;
+        }
+    }
+    // A few far refs stretch the box and coarsen every cell: when at most
+    // kMaxOutside finite refs fall outside the sampled inner box and it holds under
+    // kNarrowVolume of the box's volume, the grid covers the inner box only and
+    // those refs are kept out of the cells (keyed by every search instead).
+    for (int f = 0; f < 3; ++f) {
+        gp->in_lo[f] = -__builtin_inf();
+        gp->in_hi[f] = __builtin_inf();
+    }
+    if (any) {
+        const long long n_out = bb[kBoxOutCount];
+        double nlo[3] = {0.0, 0.0, 0.0}, next[3] = {0.0, 0.0, 0.0};
+        bool tight = true;  // some ref lies inside the inner box
+        for (int f = 0; f < d; ++f) tight = tight && bb[kBoxTight + 2 * f] <= bb[kBoxTight + 2 * f + 1];
+        if (n_out > 0 && n_out <= kMaxOutside && tight &&
+            inner_box_ratio(bb, d, nlo, next) < kNarrowVolume) {
+            // the grid spans the refs inside the inner box; the inner box decides
+            // which points stay out
+            for (int f = 0; f < d; ++f) {
+                lo[f] = key_dbl(bb[kBoxTight + 2 * f]);
+                ext[f] = key_dbl(bb[kBoxTight + 2 * f + 1]) - lo[f];
+                gp->in_lo[f] = key_dbl(bb[kBoxInner + 2 * f]);
+                gp->in_hi[f] = key_dbl(bb[kBoxInner + 2 * f + 1]);
+            }
         }
     }
     for (int f = 0; f < 3; ++f)
@@ -320,8 +493,13 @@ __global__ __launch_bounds__(256) void sort_count_kernel(SortArgs a) {
     const int d = a.d;
     const double* u = a.u64 + (J.row0 + i) * d;
     const GridParams* gp = J.gp;
-    int b = J.nb - 1;
-    if (point_finite(u, d)) {
+    int b = J.nb - 1;  // non-finite
+    bool out = false;
+    if (point_finite(u, d))
+        for (int f = 0; f < d; ++f) out = out || u[f] < gp->in_lo[f] || u[f] > gp->in_hi[f];
+    if (out) {
+        b = J.nb - 2;  // outside the grid's box
+    } else if (point_finite(u, d)) {
         const double ih = gp->inv_h;
         int c[3] = {0, 0, 0};
 #pragma unroll
@@ -496,6 +674,9 @@ struct BandDesc {
     const uint64_t* rowmask;    // occupancy bits of the ref cells
     const GridPt* rpts;         // refs in cell order
     const GridPt* qpts;         // the band's queries in tile order
+    const int32_t* qstart;      // the band's query bucket starts
+    int32_t qfar;               // bucket of its queries outside the grid's box
+    int32_t ref_out;            // bucket of the grid's refs outside its box
     int64_t n_q;
     int64_t wave_off;
 };
@@ -592,7 +773,9 @@ void lane_knn_kernel(SearchArgs a) {
         NARROW)
         return;
     const int64_t j = (wg - bd0.wave_off) * 64 + lane;
-    const bool jv = j < bd0.n_q;
+    // queries outside the grid's box are searched one per wave (wave_far_kernel)
+    const int32_t far0 = bd0.qstart[bd0.qfar], far1 = bd0.qstart[bd0.qfar + 1];
+    const bool jv = j < bd0.n_q && !(j >= far0 && j < far1);
     double qu[3] = {0.0, 0.0, 0.0};
     int64_t q = 0;
     if (jv) {
@@ -759,7 +942,12 @@ void lane_knn_kernel(SearchArgs a) {
             const uint64_t sel = wdt >= 64 ? ~0ull : ((1ull << wdt) - 1ull);
             return ((m >> (a & 63)) & sel) != 0ull;
         };
+        // the refs outside the grid's box first (every lane keys them all)
         int32_t p = 0, e = 0;
+        if (active) {
+            p = cs[bd0.ref_out];
+            e = cs[bd0.ref_out + 1];
+        }
         bool ring_end = false;   // ring walked: wait for the merge, then the stop test
         bool done = !active;
 #pragma unroll 1
@@ -960,16 +1148,9 @@ constexpr int kWaveInsertMax = 6;
 // covers the grid.  Rings whose interior rows have both end cells off the grid
 // (wider than the grid in x) enumerate their boundary rows only.
 template <int D>
-__global__ __launch_bounds__(256) void wave_knn_kernel(SearchArgs a) {
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    const int64_t gq = (int64_t)blockIdx.x * 4 + w;  // query slot over all bands
-    if (gq >= a.n_waves) return;
-    int bj = 0;
-    for (int t = 1; t < a.n_bands; ++t)
-        if (a.band[t].wave_off <= gq) bj = t;
-    const BandDesc& bd = a.band[bj];
-    const GridPt qp = bd.qpts[gq - bd.wave_off];
+__device__ __forceinline__ void wave_search_query(const SearchArgs& a, const BandDesc& bd,
+                                                  int64_t jq, int lane) {
+    const GridPt qp = bd.qpts[jq];
     double qu[3];
 #pragma unroll
     for (int f = 0; f < 3; ++f) qu[f] = qp.u[f];
@@ -1015,6 +1196,74 @@ __global__ __launch_bounds__(256) void wave_knn_kernel(SearchArgs a) {
             }
         };
         set_threshold();
+        // key the ref at position pos of the grid's point array (pv: the lane holds
+        // one) and admit it
+        auto consume = [&](int pos, bool pv) {
+            const GridPt* rq = rpts + pos;
+            const u32x4 w01 = *reinterpret_cast<const u32x4*>(rq);
+            const u32x3 w2 = *reinterpret_cast<const u32x3*>(&rq->u[2]);
+            double ru[3];
+            ru[0] = __hiloint2double((int)w01.y, (int)w01.x);
+            ru[1] = __hiloint2double((int)w01.w, (int)w01.z);
+            ru[2] = __hiloint2double((int)w2.y, (int)w2.x);
+            const int ri = (int)w2.z;
+            const double t0 = ru[0] - qu[0];
+            double key = t0 * t0;
+#pragma unroll
+            for (int f = 1; f < D; ++f) {
+                const double tt = ru[f] - qu[f];
+                key = fma(tt, tt, key);
+            }
+            const bool adm = pv & (ri < lim32) & pair_less(key, ri, tk, ti);
+            uint64_t bal = __builtin_amdgcn_ballot_w64(adm);
+            if (!bal) return;
+            if (__builtin_popcountll(bal) <= kWaveInsertMax) {
+                do {
+                    const int s = (int)__builtin_ctzll(bal);
+                    bal &= bal - 1;
+                    const double ck = lane_d(key, s);
+                    const int ci = __builtin_amdgcn_readlane(ri, s);
+                    if (!pair_less(ck, ci, tk, ti)) continue;
+                    const uint64_t m = __builtin_amdgcn_ballot_w64(pair_less(ck, ci, bk, bi));
+                    const int at = (int)__builtin_ctzll(m);
+                    const double pk = __shfl_up(bk, 1, 64);
+                    const int pi = __shfl_up(bi, 1, 64);
+                    if (lane > at) {
+                        bk = pk;
+                        bi = pi;
+                    } else if (lane == at) {
+                        bk = ck;
+                        bi = ci;
+                    }
+                    set_threshold();
+                } while (bal);
+            } else {
+                double sk = adm ? key : __builtin_inf();
+                int si = adm ? ri : -1;
+                for (int size = 2; size <= 64; size <<= 1)
+                    for (int stride = size >> 1; stride > 0; stride >>= 1)
+                        wave_pair_step(sk, si, lane, size, stride);
+                const double rk = __shfl(sk, 63 - lane, 64);
+                const int rix = __shfl(si, 63 - lane, 64);
+                if (pair_less(rk, rix, bk, bi)) {
+                    bk = rk;
+                    bi = rix;
+                }
+                for (int stride = 32; stride > 0; stride >>= 1)
+                    wave_pair_step(bk, bi, lane, 64, stride);
+                set_threshold();
+            }
+        };
+        // the refs outside the grid's box first
+        {
+            const int32_t os = cs[bd.ref_out], oe = cs[bd.ref_out + 1];
+#pragma unroll 1
+            for (int32_t cb = os; cb < oe; cb += 64) {
+                const int32_t pos = cb + lane;
+                consume(pos < oe ? pos : os, pos < oe);
+            }
+            examined += (uint32_t)(oe - os);
+        }
 #pragma unroll 1
         for (int R = 0;; ++R) {
             const int y0 = max(c[1] - R, 0), y1 = min(c[1] + R, g[1] - 1);
@@ -1130,60 +1379,7 @@ __global__ __launch_bounds__(256) void wave_knn_kernel(SearchArgs a) {
                     const int sp1 = __shfl(p1, src, 64);
                     const bool pv = Pn < T;
                     const int pos = pv ? (o < sn0 ? sp0 + o : sp1 + (o - sn0)) : 0;
-                    const GridPt* rq = rpts + pos;
-                    const u32x4 w01 = *reinterpret_cast<const u32x4*>(rq);
-                    const u32x3 w2 = *reinterpret_cast<const u32x3*>(&rq->u[2]);
-                    double ru[3];
-                    ru[0] = __hiloint2double((int)w01.y, (int)w01.x);
-                    ru[1] = __hiloint2double((int)w01.w, (int)w01.z);
-                    ru[2] = __hiloint2double((int)w2.y, (int)w2.x);
-                    const int ri = (int)w2.z;
-                    const double t0 = ru[0] - qu[0];
-                    double key = t0 * t0;
-#pragma unroll
-                    for (int f = 1; f < D; ++f) {
-                        const double tt = ru[f] - qu[f];
-                        key = fma(tt, tt, key);
-                    }
-                    const bool adm = pv & (ri < lim32) & pair_less(key, ri, tk, ti);
-                    uint64_t bal = __builtin_amdgcn_ballot_w64(adm);
-                    if (!bal) continue;
-                    if (__builtin_popcountll(bal) <= kWaveInsertMax) {
-                        do {
-                            const int s = (int)__builtin_ctzll(bal);
-                            bal &= bal - 1;
-                            const double ck = lane_d(key, s);
-                            const int ci = __builtin_amdgcn_readlane(ri, s);
-                            if (!pair_less(ck, ci, tk, ti)) continue;
-                            const uint64_t m = __builtin_amdgcn_ballot_w64(pair_less(ck, ci, bk, bi));
-                            const int at = (int)__builtin_ctzll(m);
-                            const double pk = __shfl_up(bk, 1, 64);
-                            const int pi = __shfl_up(bi, 1, 64);
-                            if (lane > at) {
-                                bk = pk;
-                                bi = pi;
-                            } else if (lane == at) {
-                                bk = ck;
-                                bi = ci;
-                            }
-                            set_threshold();
-                        } while (bal);
-                    } else {
-                        double sk = adm ? key : __builtin_inf();
-                        int si = adm ? ri : -1;
-                        for (int size = 2; size <= 64; size <<= 1)
-                            for (int stride = size >> 1; stride > 0; stride >>= 1)
-                                wave_pair_step(sk, si, lane, size, stride);
-                        const double rk = __shfl(sk, 63 - lane, 64);
-                        const int rix = __shfl(si, 63 - lane, 64);
-                        if (pair_less(rk, rix, bk, bi)) {
-                            bk = rk;
-                            bi = rix;
-                        }
-                        for (int stride = 32; stride > 0; stride >>= 1)
-                            wave_pair_step(bk, bi, lane, 64, stride);
-                        set_threshold();
-                    }
+                    consume(pos, pv);
                 }
             }
             // stop test: the nearest face of the visited block with cells beyond it
@@ -1203,8 +1399,55 @@ __global__ __launch_bounds__(256) void wave_knn_kernel(SearchArgs a) {
     if (a.pair_count && lane == 0 && examined) atomicAdd(a.pair_count, (unsigned long long)examined);
 }
 
-// One launch over every band's queries, one wave each (band table: wave_off =
-// first query slot of the band).
+// Every query of every band, one per wave (band table: wave_off = first query slot
+// of the band).
+template <int D>
+__global__ __launch_bounds__(256) void wave_knn_kernel(SearchArgs a) {
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int64_t gq = (int64_t)blockIdx.x * 4 + w;  // query slot over all bands
+    if (gq >= a.n_waves) return;
+    int bj = 0;
+    for (int t = 1; t < a.n_bands; ++t)
+        if (a.band[t].wave_off <= gq) bj = t;
+    wave_search_query<D>(a, a.band[bj], gq - a.band[bj].wave_off, lane);
+}
+
+// The queries outside the grid's box (each band's far bucket, counts on the
+// device), one per wave, the waves striding over them; after lane_knn_kernel,
+// which leaves them out.
+template <int D>
+__global__ __launch_bounds__(256) void wave_far_kernel(SearchArgs a) {
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    // lane t: band t's far queries [f0, f0 + cnt) and the inclusive prefix of the
+    // counts (bands <= kMaxGridBands + 1 <= 64)
+    int32_t f0 = 0, cnt = 0;
+    if (lane < a.n_bands) {
+        const BandDesc& b = a.band[lane];
+        f0 = b.qstart[b.qfar];
+        cnt = b.qstart[b.qfar + 1] - f0;
+    }
+    int32_t incl = cnt;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int32_t o = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += o;
+    }
+    const int64_t total = __builtin_amdgcn_readlane(incl, 63);
+    const int64_t stride = (int64_t)gridDim.x * 4;
+#pragma unroll 1
+    for (int64_t s = (int64_t)blockIdx.x * 4 + w; s < total; s += stride) {
+        // the first band whose inclusive prefix exceeds s
+        const uint64_t m = __builtin_amdgcn_ballot_w64((int64_t)incl > s);
+        const int bj = (int)__builtin_ctzll(m);
+        const int64_t before = __builtin_amdgcn_readlane(incl - cnt, bj);
+        const int64_t jq = __builtin_amdgcn_readlane(f0, bj) + (s - before);
+        wave_search_query<D>(a, a.band[bj], jq, lane);
+    }
+}
+
+// One launch over every band's queries, one wave each.
 void launch_wave_search(SearchArgs& c, int d, hipStream_t st) {
     int64_t nq = 0;
     for (int t = 0; t < c.n_bands; ++t) {
@@ -1233,8 +1476,9 @@ __global__ __launch_bounds__(64 * kCoopWaves) void wave_cost_kernel(SearchArgs a
         if (a.band[t].wave_off <= wg) bj = t;
     const BandDesc& bd = a.band[bj];
     const int64_t j = (wg - bd.wave_off) * 64 + lane;
+    const int32_t far0 = bd.qstart[bd.qfar], far1 = bd.qstart[bd.qfar + 1];
     int c = 0;
-    if (j < bd.n_q) {
+    if (j < bd.n_q && !(j >= far0 && j < far1)) {
         const GridPt qp = bd.qpts[j];
         const GridParams& P = *bd.gp;
         bool fin = true;
@@ -1309,6 +1553,11 @@ void launch_search(SearchArgs& c, int d, const GridSearchMem& M, hipStream_t st)
     hipLaunchKernelGGL(wave_sort_kernel, dim3(1), dim3(1024), 0, st, M.wave_cost, waves,
                        M.wave_perm);
     const int K = c.k <= 16 ? 16 : (c.k <= 32 ? 32 : 64);
+    // the queries outside the grid's box, which the lane kernel leaves out: their
+    // count is on the device, so the waves (at most 512) stride over them
+    int64_t nq = 0;
+    for (int t = 0; t < c.n_bands; ++t) nq += c.band[t].n_q;
+    const unsigned fb = (unsigned)std::min<int64_t>((nq + 3) / 4, 128);
     switch (d * 100 + K) {
         case 116: launch_search_dk<1, 16>(c, blocks, st); break;
         case 132: launch_search_dk<1, 32>(c, blocks, st); break;
@@ -1320,6 +1569,9 @@ void launch_search(SearchArgs& c, int d, const GridSearchMem& M, hipStream_t st)
         case 332: launch_search_dk<3, 32>(c, blocks, st); break;
         default: launch_search_dk<3, 64>(c, blocks, st); break;
     }
+    if (d == 1) hipLaunchKernelGGL(wave_far_kernel<1>, dim3(fb), dim3(256), 0, st, c);
+    else if (d == 2) hipLaunchKernelGGL(wave_far_kernel<2>, dim3(fb), dim3(256), 0, st, c);
+    else hipLaunchKernelGGL(wave_far_kernel<3>, dim3(fb), dim3(256), 0, st, c);
 }
 
 int grid_cap(int64_t n_pts) {
@@ -1332,7 +1584,7 @@ int grid_capq(int cap) { return 3 * cap + 64; }
 
 void carve_grid(Carver& c, int64_t n_pts, GridMem* M) {
     const int cap = grid_cap(n_pts);
-    const int64_t ns = (int64_t)cap + 2;
+    const int64_t ns = (int64_t)cap + 3;
     M->cap = cap;
     M->params = c.take<GridParams>(1);
     M->cnt = c.take<int32_t>((size_t)ns);
@@ -1357,7 +1609,7 @@ SortJob ref_job(const GridMem& M, int64_t n_pts) {
     J.rowmask = M.rowmask;
     J.n = n_pts;
     J.row0 = 0;
-    J.nb = M.cap + 1;
+    J.nb = M.cap + 2;
     J.tiled = 0;
     J.cap = M.cap;
     J.capq = grid_capq(M.cap);
@@ -1379,7 +1631,7 @@ SortJob query_job(const GridMem& M, int64_t qa, int64_t qb, int64_t pos,
     J.rowmask = nullptr;
     J.n = qb - qa;
     J.row0 = qa;
-    J.nb = bins.capq + 1;
+    J.nb = bins.capq + 2;
     J.tiled = 1;
     J.cap = M.cap;
     J.capq = bins.capq;
@@ -1389,12 +1641,15 @@ SortJob query_job(const GridMem& M, int64_t qa, int64_t qb, int64_t pos,
     band->rowmask = M.rowmask;
     band->rpts = M.pts;
     band->qpts = S.qpts + pos;
+    band->qstart = bins.start;
+    band->qfar = bins.capq;
+    band->ref_out = M.cap;
     band->n_q = qb - qa;
     return J;
 }
 
 void carve_bins(Carver& c, int capq, GridQueryBins* Q) {
-    const int64_t ns = (int64_t)capq + 2;
+    const int64_t ns = (int64_t)capq + 3;
     Q->capq = capq;
     Q->cnt = c.take<int32_t>((size_t)ns);
     Q->start = c.take<int32_t>((size_t)ns);
@@ -1412,7 +1667,7 @@ int64_t grid_prefix_limit(int64_t n_ref, int k) {
 }
 
 void carve_grid_refs(Carver& c, int64_t n_ref, GridRefMem* M) {
-    M->bbox = c.take<long long>(8 + 6 * (size_t)kBBoxBlocks);  // the box, then block partials
+    M->bbox = c.take<long long>(kBoxPartials + 6 * (size_t)kBBoxBlocks);  // boxes, then block partials
     carve_grid(c, n_ref, &M->full);
 }
 
@@ -1441,11 +1696,17 @@ void carve_grid_search(Carver& c, int64_t n_ref, int64_t n_query, int k, bool wi
 int grid_build(const double* u64, int64_t n, int d, const GridRefMem& M, hipStream_t st) {
     if (d < 1 || d > 3) return NNGP_ERR_INVALID_ARGUMENT;
     timing_begin(TK_KNN_GRID_BUILD, st);
-    long long* part = M.bbox + 8;
+    long long* part = M.bbox + kBoxPartials;
     const int gb = (int)std::min<int64_t>(kBBoxBlocks, (n + 255) / 256);
     if (gb > 0)
         hipLaunchKernelGGL(grid_bbox_kernel, dim3((unsigned)gb), dim3(256), 0, st, u64, n, d, part);
     hipLaunchKernelGGL(grid_bbox_final_kernel, dim3(1), dim3(256), 0, st, part, gb, M.bbox);
+    if (n > 0) {
+        hipLaunchKernelGGL(grid_box_sample_kernel, dim3(1), dim3(kBoxSample), 0, st, u64, n, d,
+                           M.bbox);
+        hipLaunchKernelGGL(grid_outside_count_kernel, dim3((unsigned)gb), dim3(256), 0, st, u64, n,
+                           d, M.bbox);
+    }
     const SortJob J = ref_job(M.full, n);
     run_sorts(u64, M.bbox, d, &J, 1, true, st);
     timing_end(TK_KNN_GRID_BUILD, st);

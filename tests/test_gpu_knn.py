@@ -187,6 +187,31 @@ def test_methods_agree_on_road_mixture_prefix():
         np.testing.assert_array_equal(outs[0][r], e)
 
 
+@pytest.mark.parametrize("d", [1, 2, 3])
+def test_grid_one_far_point_large(d):
+    """100,000 uniform points and one far one (the lane-per-query kernel's size):
+    the far point no longer coarsens the grid (the search took 22 ms with it and
+    0.65 ms without it at d = 3); exact on the far row, the rows after it and a
+    sample of rows, each against its own prefix."""
+    import time
+    rng = np.random.default_rng(40 + d)
+    n, k = 100_001, 32
+    x = rng.random((n, d))
+    x[n // 3] = 1000.0
+    xt = t(x)
+    with hip_ops.knn_method(hip_ops.KNN_AUTO):
+        got = hip_ops.knn(xt, xt, 0.1, k, ref_cut=0).cpu().numpy()
+        rows = np.concatenate([[n // 3, n // 3 + 1, n - 1], rng.choice(n, 30)])
+        for r in rows:
+            np.testing.assert_array_equal(got[r], vo.knn(x[:r], x[r:r + 1], 0.1, k)[0])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            hip_ops.knn(xt, xt, 0.1, k, ref_cut=0)
+        torch.cuda.synchronize()
+        assert (time.perf_counter() - t0) / 5 < 0.006
+
+
 @pytest.mark.parametrize("ls", [[1e5, 1e5, 1e-5], [731.42, 302.17, 0.012312]])
 def test_strongly_anisotropic_scales(ls):
     """Length scales that put the scaled points on (nearly) a line: the grid is one
@@ -242,12 +267,16 @@ def test_screen_with_outliers(case, cut, screen):
     assert rec <= limit, rec
 
 
+@pytest.mark.parametrize("grid", [hip_ops.KNN_GRID_WAVES, hip_ops.KNN_GRID_LANES],
+                         ids=["waves", "lanes"])
 @pytest.mark.parametrize("cut", [None, 0])
 @pytest.mark.parametrize("case", ["one_far_point", "few_far_points", "far_cluster", "heavy_tails"])
-def test_grid_with_outliers(case, cut):
-    """Uniform-grid search (d = 3) when a few points lie far out (the box they
-    stretch leaves most refs in a few cells): indices stay exact for the far rows,
-    the rows around them, and separate queries far outside on every side."""
+def test_grid_with_outliers(case, cut, grid):
+    """Uniform-grid search (d = 3) when a few points lie far out (the grid then
+    covers a box sampled from the refs and keeps the far refs out of its cells;
+    every search keys them, and far queries are searched one per wave): indices stay
+    exact for the far rows, the rows around them, separate queries far outside on
+    every side and queries next to the far refs, with both grid kernels."""
     rng = np.random.default_rng(10 + ["one_far_point", "few_far_points", "far_cluster",
                                       "heavy_tails"].index(case))
     n, k = 5000, 16
@@ -263,12 +292,13 @@ def test_grid_with_outliers(case, cut):
         x[n // 3: n // 3 + 50] = x[n // 3: n // 3 + 50] * 0.01 + 100.0
     else:
         x = rng.standard_t(1.5, size=(n, 3))
-    with hip_ops.knn_method(hip_ops.KNN_AUTO):
+    with hip_ops.knn_method(grid):
         got = hip_knn_self(x, 0.1, k, cut, 1)
         np.testing.assert_array_equal(got, vo.knn(x, x, 0.1, k, ref_cut=cut))
         if cut is None:
+            far = np.nonzero(np.abs(x - 0.5).max(axis=1) > 2.0)[0][:20]
             xq = np.concatenate([rng.random((200, 3)), rng.normal(0.0, 300.0, (100, 3)),
-                                 x[rng.choice(n, 50)] + 1e-4])
+                                 x[rng.choice(n, 50)] + 1e-4, x[far] + 1e-3])
             np.testing.assert_array_equal(hip_knn(x, xq, 0.1, k, None, 0),
                                           vo.knn(x, xq, 0.1, k))
 
