@@ -158,6 +158,42 @@ __device__ __forceinline__ void wave_pair_step(double& key, int& idx, int lane, 
     }
 }
 
+// Value of lane l ^ stride (stride a power of two below 64, a constant once
+// unrolled): DPP moves for strides 1, 2 and 8 (quad permutes, a rotation by 8 within
+// each row of 16), ds_swizzle xor patterns for 4 and 16, a permute for 32.
+__device__ __forceinline__ int lane_xor(int v, int stride) {
+    switch (stride) {
+        case 1: return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xf, 0xf, true);   // quad_perm [1,0,3,2]
+        case 2: return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xf, 0xf, true);   // quad_perm [2,3,0,1]
+        case 4: return __builtin_amdgcn_ds_swizzle(v, 0x101F);              // xor 4 (32-lane groups)
+        case 8: return __builtin_amdgcn_mov_dpp(v, 0x128, 0xf, 0xf, true);  // row_ror:8
+        case 16: return __builtin_amdgcn_ds_swizzle(v, 0x401F);             // xor 16 (32-lane groups)
+        default: return __shfl_xor(v, 32, 64);
+    }
+}
+__device__ __forceinline__ double lane_xor(double v, int stride) {
+    const long long b = __double_as_longlong(v);
+    const int lo = lane_xor((int)(b & 0xffffffffLL), stride);
+    const int hi = lane_xor((int)(b >> 32), stride);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// wave_pair_step with the exchange of lane_xor; size and stride must be constants
+// once unrolled.
+__device__ __forceinline__ void wave_pair_step_x(double& key, int& idx, int lane, int size,
+                                                 int stride) {
+    const double ok = lane_xor(key, stride);
+    const int oi = lane_xor(idx, stride);
+    const bool up = (lane & size) == 0;
+    const bool lower = (lane & stride) == 0;
+    const bool other_less = pair_less(ok, oi, key, idx);
+    const bool other_more = pair_less(key, idx, ok, oi);
+    if (lower == up ? other_less : other_more) {
+        key = ok;
+        idx = oi;
+    }
+}
+
 // FP64 key of a (ref, query) pair: fma(t_{d-1}, t_{d-1}, ... fma(t_1, t_1, t_0 * t_0)),
 // t_f = u_ref,f - u_query,f.
 __device__ __forceinline__ double exact_key(const double* ur, const double* uq, int d) {

@@ -1240,17 +1240,20 @@ __device__ __forceinline__ void wave_search_query(const SearchArgs& a, const Ban
             } else {
                 double sk = adm ? key : __builtin_inf();
                 int si = adm ? ri : -1;
+#pragma unroll
                 for (int size = 2; size <= 64; size <<= 1)
+#pragma unroll
                     for (int stride = size >> 1; stride > 0; stride >>= 1)
-                        wave_pair_step(sk, si, lane, size, stride);
+                        wave_pair_step_x(sk, si, lane, size, stride);
                 const double rk = __shfl(sk, 63 - lane, 64);
                 const int rix = __shfl(si, 63 - lane, 64);
                 if (pair_less(rk, rix, bk, bi)) {
                     bk = rk;
                     bi = rix;
                 }
+#pragma unroll
                 for (int stride = 32; stride > 0; stride >>= 1)
-                    wave_pair_step(bk, bi, lane, 64, stride);
+                    wave_pair_step_x(bk, bi, lane, 64, stride);
                 set_threshold();
             }
         };
