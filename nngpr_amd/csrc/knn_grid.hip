@@ -63,7 +63,8 @@ namespace {
 constexpr int kPointsPerCell = 2;
 constexpr int kScanBlock = 1024;   // items per block of the exclusive scan (256 x 4)
 constexpr int kCoopWaves = 4;      // waves per search block
-constexpr int kCoopSlots = 16;     // LDS append slots per lane
+constexpr int kCoopSlots = 16;     // LDS append slots per lane (lists of 16)
+constexpr int kCoopSlotsWide = 24; // LDS append slots per lane (lists of 32 and 64)
 // Searches whose full grid (the top band's) is at most this many cells wide in x
 // take the instantiation that walks only the boundary rows of rings wider than the
 // x extent.
@@ -753,7 +754,7 @@ __global__ __launch_bounds__(64 * kCoopWaves)
 // spilling to scratch; 16 % faster on the Vecchia search, 40 % on a query search.)
 __attribute__((amdgpu_waves_per_eu(K <= 16 ? 3 : (K <= 32 ? 2 : 1))))
 void lane_knn_kernel(SearchArgs a) {
-    constexpr int B = kCoopSlots;
+    constexpr int B = K == 16 ? kCoopSlots : kCoopSlotsWide;
     // refs per step of a lane, their loads in flight together (4: 2.14 ms, 8: 1.97 ms,
     // 12: 2.06 ms, 16: 2.62 ms on the 434,874-row Vecchia search)
     constexpr int CH = 8;
@@ -955,8 +956,11 @@ void lane_knn_kernel(SearchArgs a) {
             if (__builtin_amdgcn_ballot_w64((cnt > B - CH) | (ring_end & (cnt > 0)))) {
                 const int nb = wave_max32(cnt);
                 // K = 16: insertion ladders (the networks would not fit its 3-wave
-                // register budget); K = 32: one network over all slots; K = 64:
-                // two halves (limits its spills)
+                // register budget); K = 32 and 64: slices of 8 slots through one
+                // network instance (24 slots, merged when a lane holds more than 16:
+                // 1.99 -> 1.91 ms at K = 32 and 6.14 -> 5.83 ms at K = 64 on the
+                // 434,874-row Vecchia search, against 16 slots merged above 8; a
+                // 16-slot network beside the 8-slot one spills)
                 if (K == 16) {
 #pragma unroll 1
                     for (int b = 0; b < nb; ++b) {
@@ -965,13 +969,9 @@ void lane_knn_kernel(SearchArgs a) {
                         const int xi = bi[b][lane];
                         topk_insert<double, K>(keys, ids, v ? x : __builtin_inf(), v ? xi : -1);
                     }
-                } else if (nb <= B / 2) {
-                    merge_slots<K, B / 2>(keys, ids, bk, bi, lane, 0, cnt);
-                } else if (K == 32) {
-                    merge_slots<K, (K == 32 ? B : B / 2)>(keys, ids, bk, bi, lane, 0, cnt);
                 } else {
-                    merge_slots<K, B / 2>(keys, ids, bk, bi, lane, 0, cnt);
-                    merge_slots<K, B / 2>(keys, ids, bk, bi, lane, B / 2, cnt);
+#pragma unroll 1
+                    for (int o = 0; o < nb; o += 8) merge_slots<K, 8>(keys, ids, bk, bi, lane, o, cnt);
                 }
                 cnt = 0;
                 if (active) {
