@@ -370,7 +370,7 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
                         u = sqdist(ard_c, X + s.i * DM, s.j);
                     }
                     const double Ev = (vi && vj && kp.n_ls > 0 && s.i != s.j)
-                                          ? (MAT ? cov_e(kp.fam, u) : exp(-u))
+                                          ? (MAT ? cov_e(kp.fam, u) : exp_neg(u))
                                           : (s.i == s.j ? 1.0 : 0.0);
                     if (e <= MR) {
                         if (s.j < s.i) Kt[stri(s.i, s.j)] = (vi && vj) ? fma(kp.s, Ev, kp.C) : 0.0;
