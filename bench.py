@@ -106,7 +106,7 @@ def c3_fit(steps=50):
     est.log_marginal_likelihood(theta, eval_gradient=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    reps = 10
+    reps = 30
     for r in range(reps):
         est.log_marginal_likelihood(theta + 1e-3 * r, eval_gradient=True)
     torch.cuda.synchronize()
