@@ -62,7 +62,11 @@ namespace {
 
 constexpr int kPointsPerCell = 2;
 constexpr int kScanBlock = 1024;   // items per block of the exclusive scan (256 x 4)
-constexpr int kCoopWaves = 4;      // waves per search block
+constexpr int kCoopWaves = 4;      // waves per block of the cost pass
+// Waves per block of the lane-per-query search: one, so that a finished wave frees
+// its slot at once (with 4, a block's slots stayed held until its slowest wave
+// ended: 66 % of the wave slots busy over the 434,874-row Vecchia search)
+constexpr int kLaneWaves = 1;
 constexpr int kCoopSlots = 16;     // LDS append slots per lane (lists of 16)
 constexpr int kCoopSlotsWide = 24; // LDS append slots per lane (lists of 32 and 64)
 // Searches whose full grid (the top band's) is at most this many cells wide in x
@@ -749,7 +753,7 @@ __device__ __forceinline__ void merge_slots(double (&keys)[K], int (&ids)[K],
 // full grid is wider than kNarrowCells in x, NARROW = true the others (every wave of
 // the instantiation that does not serve the search returns at once).
 template <int D, int K, bool NARROW>
-__global__ __launch_bounds__(64 * kCoopWaves)
+__global__ __launch_bounds__(64 * kLaneWaves)
 // (K = 64: one wave per SIMD, so that the lists live in VGPRs and AGPRs instead of
 // spilling to scratch; 16 % faster on the Vecchia search, 40 % on a query search.)
 __attribute__((amdgpu_waves_per_eu(K <= 16 ? 3 : (K <= 32 ? 2 : 1))))
@@ -758,11 +762,11 @@ void lane_knn_kernel(SearchArgs a) {
     // refs per step of a lane, their loads in flight together (4: 2.14 ms, 8: 1.97 ms,
     // 12: 2.06 ms, 16: 2.62 ms on the 434,874-row Vecchia search)
     constexpr int CH = 8;
-    __shared__ double sk[kCoopWaves][B][64];
-    __shared__ int si[kCoopWaves][B][64];
+    __shared__ double sk[kLaneWaves][B][64];
+    __shared__ int si[kLaneWaves][B][64];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
     const int lane = threadIdx.x & 63;
-    const int64_t slot = (int64_t)blockIdx.x * kCoopWaves + w;
+    const int64_t slot = (int64_t)blockIdx.x * kLaneWaves + w;
     // slots past the last wave map past every band (no query)
     const int64_t wg = slot < a.n_waves ? (int64_t)__builtin_amdgcn_readfirstlane(a.perm[slot])
                                         : a.n_waves;
@@ -1532,10 +1536,11 @@ __global__ __launch_bounds__(1024) void wave_sort_kernel(const int32_t* cost, in
 }
 
 template <int D, int K>
-void launch_search_dk(const SearchArgs& c, unsigned blocks, hipStream_t st) {
-    hipLaunchKernelGGL((lane_knn_kernel<D, K, false>), dim3(blocks), dim3(64 * kCoopWaves), 0, st,
+void launch_search_dk(const SearchArgs& c, hipStream_t st) {
+    const unsigned blocks = (unsigned)((c.n_waves + kLaneWaves - 1) / kLaneWaves);
+    hipLaunchKernelGGL((lane_knn_kernel<D, K, false>), dim3(blocks), dim3(64 * kLaneWaves), 0, st,
                        c);
-    hipLaunchKernelGGL((lane_knn_kernel<D, K, true>), dim3(blocks), dim3(64 * kCoopWaves), 0, st,
+    hipLaunchKernelGGL((lane_knn_kernel<D, K, true>), dim3(blocks), dim3(64 * kLaneWaves), 0, st,
                        c);
 }
 
@@ -1562,15 +1567,15 @@ void launch_search(SearchArgs& c, int d, const GridSearchMem& M, hipStream_t st)
     for (int t = 0; t < c.n_bands; ++t) nq += c.band[t].n_q;
     const unsigned fb = (unsigned)std::min<int64_t>((nq + 3) / 4, 128);
     switch (d * 100 + K) {
-        case 116: launch_search_dk<1, 16>(c, blocks, st); break;
-        case 132: launch_search_dk<1, 32>(c, blocks, st); break;
-        case 164: launch_search_dk<1, 64>(c, blocks, st); break;
-        case 216: launch_search_dk<2, 16>(c, blocks, st); break;
-        case 232: launch_search_dk<2, 32>(c, blocks, st); break;
-        case 264: launch_search_dk<2, 64>(c, blocks, st); break;
-        case 316: launch_search_dk<3, 16>(c, blocks, st); break;
-        case 332: launch_search_dk<3, 32>(c, blocks, st); break;
-        default: launch_search_dk<3, 64>(c, blocks, st); break;
+        case 116: launch_search_dk<1, 16>(c, st); break;
+        case 132: launch_search_dk<1, 32>(c, st); break;
+        case 164: launch_search_dk<1, 64>(c, st); break;
+        case 216: launch_search_dk<2, 16>(c, st); break;
+        case 232: launch_search_dk<2, 32>(c, st); break;
+        case 264: launch_search_dk<2, 64>(c, st); break;
+        case 316: launch_search_dk<3, 16>(c, st); break;
+        case 332: launch_search_dk<3, 32>(c, st); break;
+        default: launch_search_dk<3, 64>(c, st); break;
     }
     if (d == 1) hipLaunchKernelGGL(wave_far_kernel<1>, dim3(fb), dim3(256), 0, st, c);
     else if (d == 2) hipLaunchKernelGGL(wave_far_kernel<2>, dim3(fb), dim3(256), 0, st, c);
