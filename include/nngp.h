@@ -150,7 +150,7 @@ int nngp_knn_rescale(const double* ref, int64_t n_ref, const double* query, int6
  * of d >= 4 runs on the matrix cores (bf16 hi/lo split products, FP32
  * accumulation, its own certified error bound).  BRUTE_FORCE_VALU: the scans,
  * with the FP32 VALU screen for every d.  The grid search runs one query per
- * lane (64 per wave) above 65,536 queries and one query per wave below; GRID_LANES
+ * lane (64 per wave) above 100,000 queries and one query per wave below; GRID_LANES
  * and GRID_WAVES are AUTO with one of the two for every query count.  Every
  * method returns the same indices.  No reference counterpart (the reference has
  * the brute-force scan only, cuda_kernel_accelerators.py:663-745).

@@ -126,7 +126,7 @@ KNN_AUTO, KNN_BRUTE_FORCE, KNN_BRUTE_FORCE_VALU, KNN_GRID_LANES, KNN_GRID_WAVES 
 
 def knn_set_method(method):
     """Process-wide kNN search method (KNN_AUTO: grid search for d <= 3, k <= 64,
-    one query per wave up to 65,536 queries and one per lane above;
+    one query per wave up to 100,000 queries and one per lane above;
     KNN_BRUTE_FORCE: always the scans; both screen d >= 4 on the matrix cores in
     FP32_REFINED; KNN_BRUTE_FORCE_VALU: the scans with the FP32 VALU screen;
     KNN_GRID_LANES / KNN_GRID_WAVES: AUTO with the one grid kernel for every

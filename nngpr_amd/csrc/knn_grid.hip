@@ -74,12 +74,14 @@ constexpr int kCoopSlotsWide = 24; // LDS append slots per lane (lists of 32 and
 // x extent.
 constexpr int kNarrowCells = 16;
 // Searches of at most this many queries run one query per wave.  A lane-per-query
-// wave walks its 64 queries' rings one after the other: ~0.5-0.7 ms on the
-// road-mixture data for any query search of 64 to 87,000 queries against its
-// 434,874 refs.  A wave per query keys 64 refs per step and ends in tens of
-// microseconds: 0.17 ms (64 queries, grid build included), 0.40 ms (43,487),
-// 0.63 ms (86,974); on the full Vecchia self-search it costs 2.7x the lane kernel.
-constexpr int64_t kWaveSearchMaxQueries = 65536;
+// wave walks its 64 queries' rings one after the other: 0.61-0.71 ms on the
+// road-mixture data for any query search of 1,024 to 87,000 queries against its
+// 434,874 refs (whole call: query binning and launches included).  A wave per query
+// keys 64 refs per step and ends in tens of microseconds, so its time grows with the
+// count: 0.19 ms (1,024 queries), 0.26 ms (16,384), 0.38 ms (43,487), 0.49 ms
+// (65,536), 0.58 ms (86,974); the two cross near 110,000.  On the full Vecchia
+// self-search a wave per query costs 2.7x the lane kernel.
+constexpr int64_t kWaveSearchMaxQueries = 100000;
 
 __device__ __forceinline__ bool point_finite(const double* u, int d) {
     bool fin = true;
@@ -696,8 +698,10 @@ struct SearchArgs {
     int64_t cut;
     int k;
     int32_t* out_idx;
-    // when non-null, every wave adds the number of (query, ref) pairs its lanes
-    // examined (the measured work of the search)
+    // when non-null, every wave of lane_knn_kernel adds the number of (query, ref)
+    // pairs its lanes examined (the measured work of the search).  The wave-per-query
+    // kernels do not count: one same-address atomic per query (43,487 of them for the
+    // road-mixture test set) cost ~0.2 ms of contention whenever timing was on.
     unsigned long long* pair_count;
 };
 
@@ -1173,7 +1177,6 @@ __device__ __forceinline__ void wave_search_query(const SearchArgs& a, const Ban
     double bk = __builtin_inf();
     asm volatile("" : "+v"(bk));
     int bi = -1;
-    uint32_t examined = 0;
     if (fin && lim > 0) {
         const GridParams& P = *bd.gp;
         int g[3], c[3];
@@ -1269,7 +1272,6 @@ __device__ __forceinline__ void wave_search_query(const SearchArgs& a, const Ban
                 const int32_t pos = cb + lane;
                 consume(pos < oe ? pos : os, pos < oe);
             }
-            examined += (uint32_t)(oe - os);
         }
 #pragma unroll 1
         for (int R = 0;; ++R) {
@@ -1366,7 +1368,6 @@ __device__ __forceinline__ void wave_search_query(const SearchArgs& a, const Ban
                 }
                 const int ex = incl - cntl;
                 const int T = __builtin_amdgcn_readlane(incl, 63);
-                examined += (uint32_t)T;
 #pragma unroll 1
                 for (int cb = 0; cb < T; cb += 64) {
                     const int Pn = cb + lane;
@@ -1403,7 +1404,6 @@ __device__ __forceinline__ void wave_search_query(const SearchArgs& a, const Ban
         }
     }
     if (lane < k) a.out_idx[q * k + lane] = bi;
-    if (a.pair_count && lane == 0 && examined) atomicAdd(a.pair_count, (unsigned long long)examined);
 }
 
 // Every query of every band, one per wave (band table: wave_off = first query slot
