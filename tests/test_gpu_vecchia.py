@@ -152,6 +152,32 @@ def test_fill_kernel_vs_oracle():
     np.testing.assert_allclose(G.cpu().numpy(), Go, rtol=1e-12, atol=1e-15)
 
 
+def test_fill_exponential_accuracy():
+    """The covariance entries' exp(-u) (the library's own FP64 routine) against
+    numpy's over u in [0, 760]: within 2 ulp in the normal range, the subnormal
+    tail within 2 of its units, and 0 from u = 746 on.  One feature, pairs (0, r)
+    with r^2 / (2 l^2) = u, s = 1 and no other kernel term, so each off-diagonal
+    entry is exactly exp(-u) of the kernel's own u."""
+    rng = np.random.default_rng(21)
+    ls = 0.5
+    u = np.concatenate([np.linspace(0.0, 760.0, 4001), rng.uniform(0.0, 50.0, 4000),
+                        rng.uniform(0.0, 1e-3, 500), rng.uniform(700.0, 750.0, 500)])
+    r = np.sqrt(2.0 * u) * ls
+    x = np.concatenate([np.zeros(1), r])[:, None]
+    nn = np.stack([np.zeros(u.size, np.int32), np.arange(1, u.size + 1, dtype=np.int32)], 1)
+    tm = {"noise_level": -1, "constant_value": -1, "rbf_level": 0, "length_scale": 1}
+    ks = kstruct(0.0, 0.0, 1.0, ls, tm, 2)
+    K, _ = hip_ops.fill_nn_kernel(t(x), t(nn, torch.int32), ks, eval_gradient=False)
+    got = K.cpu().numpy()[:, 0, 1]
+    uk = (r * r) * (1.0 / (2.0 * ls * ls))      # the kernel's own u, as it forms it
+    exp = np.exp(-uk)
+    normal = exp >= np.finfo(np.float64).tiny
+    ulp = np.spacing(exp[normal])
+    assert np.max(np.abs(got[normal] - exp[normal]) / ulp) <= 2.0
+    assert np.max(np.abs(got[~normal] - exp[~normal])) <= 2 * 5e-324
+    assert np.all(got[uk >= 746.0] == 0.0)
+
+
 def near_duplicate_set(seed=12):
     """300 points in the unit square, then a copy of each moved by about 1e-3 (rows
     300-599, so every copy conditions on its original).  With l = 0.05, White(1e-6)
