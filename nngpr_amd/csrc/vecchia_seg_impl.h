@@ -311,6 +311,9 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
     const uint64_t b0 = __ballot(v0), b1 = __ballot(v1);
     const uint32_t vlo = (uint32_t)((b0 >> (seg * S)) & ((1ull << S) - 1ull));
     const uint32_t vhi = (uint32_t)((b1 >> (seg * S)) & ((1ull << S) - 1ull));
+    // validity of row j as bit j (rows >= S from the second rows' ballot, reversed)
+    const uint64_t vmask =
+        (uint64_t)vlo | ((uint64_t)(__builtin_bitreverse32(vhi) >> (32 - S)) << S);
     __syncthreads();
 
     // per-feature weights of the squared distance (ARD: h_f; isotropic: applied after)
@@ -350,36 +353,39 @@ __device__ __forceinline__ void seg_body(const VecArgs& a, const RegExtra& ex, i
     // stationary part E of a valid off-diagonal pair from its covariance entry
     auto e_of = [&](double K) -> double { return kp.n_ls > 0 ? (K - kp.C) * inv_s : 0.0; };
     double e0s = 0.0, e1s = 0.0;
+    // Branch-free: every entry's stationary value is evaluated and then selected, and
+    // the entries that are not stored (the two diagonal ones, the two against the
+    // point, the steps past the last entry) are written to this lane's slot of the
+    // first vector, which the factorisation overwrites before it reads it.  (With
+    // the evaluation and the store behind per-lane branches the fill issued about as
+    // many scalar as vector instructions, and was 37 % of the pass.)
+    double* const spare = xv + t;
     auto fill_entries = [&](auto ard_c) {
 #pragma unroll 1
         for (int e0 = 0; e0 < NE; e0 += U) {
 #pragma unroll
             for (int uu = 0; uu < U; ++uu) {
                 const int e = e0 + uu;
-                if (e < NE) {
-                    const SegEntry s = seg_entry<MR>(e, i0, i1);
-                    const bool vi = s.first ? v0 : v1;
-                    const bool vj = s.j == MR ? true : row_valid<MR>(s.j, vlo, vhi);
-                    double u;
-                    if constexpr (DM <= 8) {
-                        double xi[DM];
+                const SegEntry s = seg_entry<MR>(e, i0, i1);
+                const bool vi = s.first ? v0 : v1;
+                const bool vj = (s.j >= MR) | (((vmask >> (s.j & 63)) & 1ull) != 0ull);
+                double u;
+                if constexpr (DM <= 8) {
+                    double xi[DM];
 #pragma unroll
-                        for (int f = 0; f < DM; ++f) xi[f] = s.first ? xr0[f] : xr1[f];
-                        u = sqdist(ard_c, xi, s.j);
-                    } else {  // wide rows: the lane's coordinates are read back from LDS
-                        u = sqdist(ard_c, X + s.i * DM, s.j);
-                    }
-                    const double Ev = (vi && vj && kp.n_ls > 0 && s.i != s.j)
-                                          ? (MAT ? cov_e(kp.fam, u) : exp_neg(u))
-                                          : (s.i == s.j ? 1.0 : 0.0);
-                    if (e <= MR) {
-                        if (s.j < s.i) Kt[stri(s.i, s.j)] = (vi && vj) ? fma(kp.s, Ev, kp.C) : 0.0;
-                    } else if (e == MR + 1) {
-                        e0s = Ev;
-                    } else {
-                        e1s = Ev;
-                    }
+                    for (int f = 0; f < DM; ++f) xi[f] = s.first ? xr0[f] : xr1[f];
+                    u = sqdist(ard_c, xi, s.j);
+                } else {  // wide rows: the lane's coordinates are read back from LDS
+                    u = sqdist(ard_c, X + s.i * DM, s.j);
                 }
+                const double Ex = MAT ? cov_e(kp.fam, u) : exp_neg(u);
+                const bool live = vi & vj & (s.i != s.j) & (kp.n_ls > 0);
+                const double Ev = live ? Ex : (s.i == s.j ? 1.0 : 0.0);
+                const bool store = (e <= MR) & (s.j < s.i);
+                double* const dst = store ? Kt + stri(s.i, s.j) : spare;
+                *dst = (vi & vj) ? fma(kp.s, Ev, kp.C) : 0.0;
+                e0s = e == MR + 1 ? Ev : e0s;
+                e1s = e == MR + 2 ? Ev : e1s;
             }
         }
     };
