@@ -322,7 +322,8 @@ int nngp_model_predict(const nngp_model* model, const double* x_query, int64_t n
 
 /*
  * Counters of the last nngp_knn / nngp_vecchia_loglik / nngp_predict call that
- * used this workspace: rows whose kNN certificate failed and were recomputed
+ * used this workspace: rows whose kNN certificate failed, or whose uniform-grid
+ * search (lists of 32 or 64) met a key equal to its k-th, and were recomputed
  * exactly, points re-factored in FP64 after the FP32 pivot check.  Synchronises
  * the stream.  Returns NNGP_ERR_INVALID_ARGUMENT when that call met neighbour
  * indices outside [-1, n), else NNGP_ERR_NOT_POSITIVE_DEFINITE when a block

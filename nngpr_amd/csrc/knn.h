@@ -86,6 +86,10 @@ struct GridSearchMem {
     // their costs
     int32_t* wave_cost;
     int32_t* wave_perm;
+    // queries the lane-per-query search queues for an exact search: (band,
+    // position) pairs, at most one per query, and their count
+    int32_t* redo;
+    int32_t* redo_n;
 };
 
 bool grid_supported(int d, int k);
@@ -100,7 +104,8 @@ int grid_build(const double* u64, int64_t n, int d, const GridRefMem& M, hipStre
 // grid_prefix_limit and are scanned elsewhere; row0 = 0 without a cut).
 int grid_search(const GridRefMem& G, const double* ru64, int64_t n_ref, int d,
                 const double* qu64, int64_t n_query, int k, int has_cut, int64_t cut,
-                int64_t row0, const GridSearchMem& M, int32_t* out_idx, hipStream_t st);
+                int64_t row0, const GridSearchMem& M, int32_t* out_idx,
+                int64_t* rows_recomputed, hipStream_t st);
 
 // ---------------------------------------------------------------------------
 // Ref-side state of the kNN search

@@ -1296,7 +1296,7 @@ int knn_search(const KnnRefs& R, const double* query, int64_t n_query, int k, in
             NNGP_HIP_CHECK(hipGetLastError());
         }
         return grid_search(R.grid, R.u64, R.n, d, qu64, n_query, k, has_cut, ref_cut, row0,
-                           Q.grid, out_idx, st);
+                           Q.grid, out_idx, hdr ? &hdr->knn_rows_recomputed : nullptr, st);
     }
     if (!fast) {
         SimpleArgs sa{};

@@ -142,6 +142,62 @@ __device__ __forceinline__ void pair_merge(double (&keys)[K], int (&ids)[K], dou
     }
 }
 
+// Key-only forms of the networks above: the index follows the key comparison
+// alone (5 instructions per compare-exchange instead of 7 plus 2 scalar mask
+// operations), so a list comes out in key order with equal keys in whatever order
+// the network left them.  key_merge also returns the smallest key it dropped:
+// when that equals the kept list's last key, the K kept pairs are right as keys
+// but which of the equal-key pairs were kept is arbitrary.
+__device__ __forceinline__ void key_cex(double& ka, int& ia, double& kb, int& ib) {
+    const bool sw = kb < ka;
+    const double lo = fmin(ka, kb), hi = fmax(ka, kb);
+    const int il = sw ? ib : ia, ih = sw ? ia : ib;
+    ka = lo;
+    kb = hi;
+    ia = il;
+    ib = ih;
+}
+
+template <int N>
+__device__ __forceinline__ void key_sort(double (&k)[N], int (&id)[N]) {
+#pragma unroll
+    for (int size = 2; size <= N; size <<= 1) {
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                const int j = i ^ stride;
+                if (j > i) {
+                    if ((i & size) == 0) key_cex(k[i], id[i], k[j], id[j]);
+                    else key_cex(k[j], id[j], k[i], id[i]);
+                }
+            }
+        }
+    }
+}
+
+template <int K, int B>
+__device__ __forceinline__ void key_merge(double (&keys)[K], int (&ids)[K], double (&bk)[B],
+                                          int (&bi)[B], double& dropped_min) {
+    key_sort<B>(bk, bi);
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+        const int t = K - B + i, j = B - 1 - i;
+        const bool lt = bk[j] < keys[t];
+        dropped_min = fmin(dropped_min, fmax(bk[j], keys[t]));
+        keys[t] = fmin(bk[j], keys[t]);
+        ids[t] = lt ? bi[j] : ids[t];
+    }
+#pragma unroll
+    for (int stride = K >> 1; stride > 0; stride >>= 1) {
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            const int j = i ^ stride;
+            if (j > i) key_cex(keys[i], ids[i], keys[j], ids[j]);
+        }
+    }
+}
+
 // Wave-wide bitonic stage over (key, index) pairs, one pair per lane: lanes l and
 // l ^ stride exchange; the lower lane of an ascending block keeps the smaller pair.
 __device__ __forceinline__ void wave_pair_step(double& key, int& idx, int lane, int size,

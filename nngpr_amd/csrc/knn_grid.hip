@@ -703,6 +703,13 @@ struct SearchArgs {
     // kernels do not count: one same-address atomic per query (43,487 of them for the
     // road-mixture test set) cost ~0.2 ms of contention whenever timing was on.
     unsigned long long* pair_count;
+    // queries lane_knn_kernel could not order exactly (equal keys at the k-th place
+    // or out of index order): (band, position) pairs and their count, searched again
+    // by wave_far_kernel; the count is cleared by wave_sort_kernel
+    int32_t* redo;
+    int32_t* redo_n;
+    // when non-null, wave_far_kernel adds the number of queued queries to it
+    int64_t* rows_recomputed;
 };
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
@@ -745,6 +752,25 @@ __device__ __forceinline__ void merge_slots(double (&keys)[K], int (&ids)[K],
     pair_merge<K, S>(keys, ids, ck, ci);
 }
 
+// As merge_slots, with the key-only network (key_merge); dropped_min collects the
+// smallest key dropped.
+template <int K, int S>
+__device__ __forceinline__ void merge_slots_keys(double (&keys)[K], int (&ids)[K],
+                                                 const double (*bk)[64], const int (*bi)[64],
+                                                 int lane, int off, int cnt, double& dropped_min) {
+    double ck[S];
+    int ci[S];
+#pragma unroll
+    for (int b = 0; b < S; ++b) {
+        const bool v = off + b < cnt;
+        const double x = bk[off + b][lane];
+        const int xi = bi[off + b][lane];
+        ck[b] = v ? x : __builtin_inf();
+        ci[b] = v ? xi : -1;
+    }
+    key_merge<K, S>(keys, ids, ck, ci, dropped_min);
+}
+
 // One lane per query, 64 neighbouring queries per wave.  A lane walks the rings
 // of cells around its own cell (R = 0, 1, ...; a ring's rows (y, z) one after the
 // other: a new row's whole x run, an old row's two new end cells) and appends the
@@ -753,6 +779,10 @@ __device__ __forceinline__ void merge_slots(double (&keys)[K], int (&ids)[K],
 // sorted register lists in lock-step (the single merge site of the loop).  After a
 // ring the lane stops once its k-th key is below the squared distance from its
 // query to the nearest cell outside the visited block.
+// K = 32 and 64 order their lists by key alone (key_merge, admission below the k-th
+// key): a lane that meets a key equal to its k-th (admitted or dropped) or ends
+// with equal keys out of index order is queued for the exact wave-per-query search
+// (wave_far_kernel), which rewrites its row.
 // Two instantiations share each launch set: NARROW = false serves searches whose
 // full grid is wider than kNarrowCells in x, NARROW = true the others (every wave of
 // the instantiation that does not serve the search returns at once).
@@ -763,6 +793,7 @@ __global__ __launch_bounds__(64 * kLaneWaves)
 __attribute__((amdgpu_waves_per_eu(K <= 16 ? 3 : (K <= 32 ? 2 : 1))))
 void lane_knn_kernel(SearchArgs a) {
     constexpr int B = K == 16 ? kCoopSlots : kCoopSlotsWide;
+    constexpr bool KEYS = K != 16;  // key-only list order (see above)
     // refs per step of a lane, their loads in flight together (4: 2.14 ms, 8: 1.97 ms,
     // 12: 2.06 ms, 16: 2.62 ms on the 434,874-row Vecchia search)
     constexpr int CH = 8;
@@ -823,6 +854,7 @@ void lane_knn_kernel(SearchArgs a) {
     init_list<K>(keys, ids, pad);
 
     uint32_t examined = 0;  // refs this lane keyed
+    bool tie = false;       // (KEYS: an equal key met at the k-th place)
     if (__builtin_amdgcn_ballot_w64(active)) {
         const double h = P.h, tol = P.tol;
         const int32_t* cs = bd0.cell_start;
@@ -832,7 +864,7 @@ void lane_knn_kernel(SearchArgs a) {
         int (*bi)[64] = si[w];
         // admission threshold pair; inactive lanes admit nothing
         double tk = active ? __builtin_inf() : -__builtin_inf();
-        int ti = -1;
+        int ti = -1;  // (K = 16: index of the threshold pair)
         // squared search radius with slack, (sqrt(tk) + tol)^2; +inf until k refs
         double rr = __builtin_inf();
         const double inv_h = P.inv_h;
@@ -978,13 +1010,17 @@ void lane_knn_kernel(SearchArgs a) {
                         topk_insert<double, K>(keys, ids, v ? x : __builtin_inf(), v ? xi : -1);
                     }
                 } else {
+                    double dmin = __builtin_inf();
 #pragma unroll 1
-                    for (int o = 0; o < nb; o += 8) merge_slots<K, 8>(keys, ids, bk, bi, lane, o, cnt);
+                    for (int o = 0; o < nb; o += 8)
+                        merge_slots_keys<K, 8>(keys, ids, bk, bi, lane, o, cnt, dmin);
+                    // dropped keys are >= the kept ones: a tie at the k-th place
+                    tie |= (dmin == keys[K - 1]) & (keys[K - 1] < __builtin_inf());
                 }
                 cnt = 0;
                 if (active) {
                     tk = keys[K - 1];
-                    ti = ids[K - 1];
+                    if (!KEYS) ti = ids[K - 1];
                     if (tk < __builtin_inf()) {
                         const double r = sqrt(tk) + tol;
                         rr = r * r;
@@ -1043,8 +1079,9 @@ void lane_knn_kernel(SearchArgs a) {
                         key = fma(tt, tt, key);
                     }
                     const int ri = rix[u];
-                    if ((p + u < e) & (ri < lim32) &
-                        ((key < tk) | ((key == tk) & (ri < ti)))) {
+                    const bool in = (p + u < e) & (ri < lim32);
+                    if (KEYS) tie |= in & (key == tk);
+                    if (in & ((key < tk) | (!KEYS & (key == tk) & (ri < ti)))) {
                         bk[cnt][lane] = key;
                         bi[cnt][lane] = ri;
                         ++cnt;
@@ -1118,6 +1155,17 @@ void lane_knn_kernel(SearchArgs a) {
     int pad_out = K - a.k;
     asm volatile("" : "+v"(pad_out));
     if (jv) store_list<K>(ids, pad_out, a.out_idx + q * a.k);
+    if (KEYS) {
+        // equal keys out of index order (placeholders share one index)
+#pragma unroll
+        for (int t = 0; t + 1 < K; ++t)
+            tie |= (keys[t] == keys[t + 1]) & ((unsigned)ids[t] > (unsigned)ids[t + 1]);
+        if (jv & tie) {
+            const int r = atomicAdd(a.redo_n, 1);
+            a.redo[2 * r] = bj;
+            a.redo[2 * r + 1] = (int32_t)j;
+        }
+    }
     if (a.pair_count) {
         unsigned long long s = examined;
 #pragma unroll
@@ -1421,8 +1469,9 @@ __global__ __launch_bounds__(256) void wave_knn_kernel(SearchArgs a) {
 }
 
 // The queries outside the grid's box (each band's far bucket, counts on the
-// device), one per wave, the waves striding over them; after lane_knn_kernel,
-// which leaves them out.
+// device), then the queries lane_knn_kernel queued for an exact search (ties at
+// the k-th key), one per wave, the waves striding over them; after
+// lane_knn_kernel, which leaves the far queries out.
 template <int D>
 __global__ __launch_bounds__(256) void wave_far_kernel(SearchArgs a) {
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1442,9 +1491,18 @@ __global__ __launch_bounds__(256) void wave_far_kernel(SearchArgs a) {
         if (lane >= off) incl += o;
     }
     const int64_t total = __builtin_amdgcn_readlane(incl, 63);
+    const int64_t redo = a.redo_n ? __builtin_amdgcn_readfirstlane(*a.redo_n) : 0;
+    if (a.rows_recomputed && blockIdx.x == 0 && threadIdx.x == 0) *a.rows_recomputed += redo;
     const int64_t stride = (int64_t)gridDim.x * 4;
 #pragma unroll 1
-    for (int64_t s = (int64_t)blockIdx.x * 4 + w; s < total; s += stride) {
+    for (int64_t s = (int64_t)blockIdx.x * 4 + w; s < total + redo; s += stride) {
+        if (s >= total) {
+            const int32_t* e = a.redo + 2 * (s - total);
+            const int bj = __builtin_amdgcn_readfirstlane(e[0]);
+            const int64_t jq = __builtin_amdgcn_readfirstlane(e[1]);
+            wave_search_query<D>(a, a.band[bj], jq, lane);
+            continue;
+        }
         // the first band whose inclusive prefix exceeds s
         const uint64_t m = __builtin_amdgcn_ballot_w64((int64_t)incl > s);
         const int bj = (int)__builtin_ctzll(m);
@@ -1512,9 +1570,10 @@ __global__ __launch_bounds__(64 * kCoopWaves) void wave_cost_kernel(SearchArgs a
 // most expensive first (order inside a bucket is arbitrary).
 constexpr int kCostBuckets = 64;
 __global__ __launch_bounds__(1024) void wave_sort_kernel(const int32_t* cost, int64_t n,
-                                                        int32_t* perm) {
+                                                        int32_t* perm, int32_t* redo_n) {
     __shared__ int hist[kCostBuckets];
     if (threadIdx.x < kCostBuckets) hist[threadIdx.x] = 0;
+    if (threadIdx.x == 0) *redo_n = 0;  // the search's redo queue starts empty
     __syncthreads();
     auto bucket = [](int c) {
         const int b = (int)(4.0f * __log2f(1.0f + (float)c));
@@ -1558,14 +1617,19 @@ void launch_search(SearchArgs& c, int d, const GridSearchMem& M, hipStream_t st)
     const unsigned blocks = (unsigned)((waves + kCoopWaves - 1) / kCoopWaves);
     hipLaunchKernelGGL(wave_cost_kernel, dim3(blocks), dim3(64 * kCoopWaves), 0, st, c,
                        M.wave_cost);
+    c.redo = M.redo;
+    c.redo_n = M.redo_n;
     hipLaunchKernelGGL(wave_sort_kernel, dim3(1), dim3(1024), 0, st, M.wave_cost, waves,
-                       M.wave_perm);
+                       M.wave_perm, M.redo_n);
     const int K = c.k <= 16 ? 16 : (c.k <= 32 ? 32 : 64);
     // the queries outside the grid's box, which the lane kernel leaves out: their
-    // count is on the device, so the waves (at most 512) stride over them
+    // count is on the device, so the waves stride over them; lists of 32 and 64
+    // also queue the queries with ties at the k-th key (on data with many equal
+    // distances, most of them): one round of the kernel's occupancy (5 waves per
+    // SIMD, 1,280 blocks of 4 waves) for those, 512 waves for 16
     int64_t nq = 0;
     for (int t = 0; t < c.n_bands; ++t) nq += c.band[t].n_q;
-    const unsigned fb = (unsigned)std::min<int64_t>((nq + 3) / 4, 128);
+    const unsigned fb = (unsigned)std::min<int64_t>((nq + 3) / 4, K == 16 ? 128 : 1280);
     switch (d * 100 + K) {
         case 116: launch_search_dk<1, 16>(c, st); break;
         case 132: launch_search_dk<1, 32>(c, st); break;
@@ -1699,6 +1763,8 @@ void carve_grid_search(Carver& c, int64_t n_ref, int64_t n_query, int k, bool wi
     const size_t max_waves = (size_t)(n_query / 64) + (size_t)nl + 2;
     M->wave_cost = c.take<int32_t>(max_waves);
     M->wave_perm = c.take<int32_t>(max_waves);
+    M->redo = c.take<int32_t>(2 * ((size_t)n_query + 1));
+    M->redo_n = c.take<int32_t>(1);
 }
 
 int grid_build(const double* u64, int64_t n, int d, const GridRefMem& M, hipStream_t st) {
@@ -1724,9 +1790,11 @@ int grid_build(const double* u64, int64_t n, int d, const GridRefMem& M, hipStre
 
 int grid_search(const GridRefMem& G, const double* ru64, int64_t n_ref, int d,
                 const double* qu64, int64_t n_query, int k, int has_cut, int64_t cut,
-                int64_t row0, const GridSearchMem& M, int32_t* out_idx, hipStream_t st) {
+                int64_t row0, const GridSearchMem& M, int32_t* out_idx,
+                int64_t* rows_recomputed, hipStream_t st) {
     if (!grid_supported(d, k)) return NNGP_ERR_UNSUPPORTED;
     SearchArgs sa{};
+    sa.rows_recomputed = rows_recomputed;
     sa.n_ref = n_ref;
     sa.has_cut = has_cut;
     sa.cut = cut;

@@ -180,10 +180,43 @@ def test_methods_agree_on_road_mixture_prefix():
             outs.append(hip_knn_self(x, 0.1, 32, 0, 1))
     for o in outs[1:]:
         np.testing.assert_array_equal(outs[0], o)
+    # no equal keys at the k-th place here: the lane kernel queues no query
+    with hip_ops.knn_method(hip_ops.KNN_GRID_LANES):
+        hip_knn_self(x, 0.1, 32, 0, 1)
+        assert hip_ops.knn_stats()["knn_rows_recomputed"] == 0
     # a sample of rows against the oracle on each row's own prefix of refs
     rows = np.concatenate([[0, 1, 31, 32, 33], np.random.default_rng(0).choice(x.shape[0], 40)])
     for r in rows:
         e = vo.knn(x[:r], x[r:r + 1], 0.1, 32)[0] if r > 0 else -np.ones(32, np.int32)
+        np.testing.assert_array_equal(outs[0][r], e)
+
+
+@pytest.mark.parametrize("k", [16, 32, 64])
+@pytest.mark.parametrize("cut", [0, None])
+def test_lattice_ties_lane_kernel(k, cut):
+    """150,000 points on an integer lattice of 60^3 sites (most keys tie): the
+    lane-per-query kernel, whose lists of 32 and 64 order keys alone and queue
+    every query with an equal key at the k-th place for the exact one-query-per-
+    wave search, gives the same indices as that search for every query, and the
+    oracle's on a sample of rows (each against its own prefix for cut 0)."""
+    rng = np.random.default_rng(11 + k)
+    x = rng.integers(0, 60, size=(150_000, 3)).astype(np.float64)
+    xt = t(x)
+    outs = []
+    for m in (hip_ops.KNN_GRID_LANES, hip_ops.KNN_GRID_WAVES):
+        with hip_ops.knn_method(m):
+            outs.append(hip_ops.knn(xt, xt, None, k, ref_cut=cut).cpu().numpy())
+            if m == hip_ops.KNN_GRID_LANES:
+                queued = hip_ops.knn_stats()["knn_rows_recomputed"]
+    np.testing.assert_array_equal(outs[0], outs[1])
+    # lists of 16 keep (key, index) order in the lane kernel and queue nothing
+    assert (queued > 10_000) if k >= 32 else (queued == 0), queued
+    rows = np.concatenate([[0, 1, k, k + 1], rng.choice(x.shape[0], 12)])
+    for r in rows:
+        if cut == 0:
+            e = vo.knn(x[:r], x[r:r + 1], None, k)[0] if r > 0 else -np.ones(k, np.int32)
+        else:
+            e = vo.knn(x, x[r:r + 1], None, k)[0]
         np.testing.assert_array_equal(outs[0][r], e)
 
 
