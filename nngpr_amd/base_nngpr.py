@@ -126,7 +126,7 @@ class HipWorker:
 
     def kernel_struct(self, kernel):
         dec = self.decompose(kernel)
-        return nat.kernel_struct(dec.params, dec.mapping, len(kernel.theta)), dec
+        return nat.kernel_struct(dec.params, dec.mapping, dec.n_theta), dec
 
     def knn_scale(self, kernel, nn_type):
         if nn_type != "kernel-space":

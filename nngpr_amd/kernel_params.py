@@ -136,13 +136,38 @@ def can_accelerate_kernel(kernel):
     return True
 
 
+def kernel_structure(kernel):
+    """What the theta mapping depends on, without the hyperparameter values: the
+    tree of kernel types, every hyperparameter's bounds (\"fixed\" removes it from
+    theta), the length-scale shapes and Matern's nu."""
+    if isinstance(kernel, (kernels.Sum, kernels.Product)):
+        return (type(kernel), kernel_structure(kernel.k1), kernel_structure(kernel.k2))
+    if isinstance(kernel, kernels.Exponentiation):
+        return (type(kernel), kernel_structure(kernel.kernel), kernel.exponent)
+    attrs = vars(kernel)
+    leaf = [type(kernel)]
+    for name in sorted(attrs):
+        v = attrs[name]
+        if name.endswith("_bounds"):
+            leaf.append((name, v if isinstance(v, str) else np.asarray(v).tobytes()))
+        elif name == "length_scale":
+            leaf.append((name, np.shape(v)))
+        elif name == "nu":
+            leaf.append((name, float(v)))
+    return tuple(leaf)
+
+
 class KernelDecomposition:
     """Cache of the decomposition of the last kernel seen (identity + theta),
-    like CudaKernelAccelerator._update_kernel_if_needed (cuda_kernel_accelerators.py:146-164)."""
+    like CudaKernelAccelerator._update_kernel_if_needed (cuda_kernel_accelerators.py:146-164).
+    The theta mapping is kept while the kernel's structure is unchanged (an
+    optimizer only moves theta), so a new theta re-reads the parameter values
+    only."""
 
     def __init__(self):
         self._kernel = None
         self._theta = None
+        self._structure = None
         self.params = None
         self.mapping = None
         self.error = None
@@ -152,15 +177,25 @@ class KernelDecomposition:
         if self._kernel is kernel and self._theta is not None and self._theta.shape == theta.shape \
                 and np.all(self._theta == theta):
             return self
+        structure = kernel_structure(kernel)
+        keep_mapping = self._kernel is kernel and self.mapping is not None and \
+            self._theta is not None and self._theta.shape == theta.shape and \
+            self._structure == structure
         self._kernel = kernel
         self._theta = copy.deepcopy(theta)
+        self._structure = structure
+        mapping = self.mapping if keep_mapping else None
         self.params, self.mapping, self.error = None, None, None
         try:
             self.params = get_kernel_params(kernel)
-            self.mapping = get_kernel_theta_mapping(kernel)
+            self.mapping = mapping if mapping is not None else get_kernel_theta_mapping(kernel)
         except NotImplementedError as e:
             self.params, self.mapping, self.error = None, None, e
         return self
+
+    @property
+    def n_theta(self):
+        return 0 if self._theta is None else int(self._theta.size)
 
     @property
     def accelerated(self):

@@ -75,6 +75,41 @@ def test_matern_other_nu_not_accelerated():
     assert not can_accelerate_kernel(skk.Matern(0.4, nu=0.5) + skk.RBF(0.2))
 
 
+def test_decomposition_cache_keeps_mapping_while_structure_holds(monkeypatch):
+    """The cached decomposition re-reads the parameter values at every new theta
+    but maps theta again only when the kernel's structure changes; each cached
+    result equals a fresh decomposition."""
+    from sklearn.gaussian_process import kernels as skk
+    from nngpr_amd import kernel_params as kp
+    calls = []
+    orig = kp.get_kernel_theta_mapping
+    monkeypatch.setattr(kp, "get_kernel_theta_mapping", lambda k: calls.append(1) or orig(k))
+
+    def same_as_fresh(dec, k):
+        p, mp = kp.get_kernel_params(k), orig(k)
+        assert dec.params.keys() == p.keys() and dec.mapping.keys() == mp.keys()
+        for a, b in ((dec.params, p), (dec.mapping, mp)):
+            for name in a:
+                np.testing.assert_array_equal(np.asarray(a[name]), np.asarray(b[name]))
+
+    k = skk.WhiteKernel(0.01) + skk.ConstantKernel(1.0) * skk.RBF([0.1, 0.2])
+    dec = kp.KernelDecomposition()
+    dec.update(k)
+    for step in range(3):  # an optimizer moving theta in place
+        k.theta = k.theta + 0.1 * (step + 1)
+        same_as_fresh(dec.update(k), k)
+    assert len(calls) == 1
+    k.set_params(k2__k1__constant_value_bounds="fixed")  # one theta fewer
+    same_as_fresh(dec.update(k), k)
+    assert len(calls) == 2
+    k.k1, k.k2 = k.k2, k.k1  # same theta count, new order
+    same_as_fresh(dec.update(k), k)
+    assert len(calls) == 3
+    other = skk.WhiteKernel(0.01) + skk.ConstantKernel(1.0) * skk.RBF([0.1, 0.2])
+    same_as_fresh(dec.update(other), other)  # another kernel object
+    assert len(calls) == 4
+
+
 def test_workspace_queries_are_host_only():
     lib = nat.lib()
     assert lib.nngp_knn_workspace_bytes(1000, 1000, 3, 16, 1) > 0
