@@ -138,7 +138,7 @@ def can_accelerate_kernel(kernel):
 
 def kernel_structure(kernel):
     """What the theta mapping depends on, without the hyperparameter values: the
-    tree of kernel types, every hyperparameter's bounds (\"fixed\" removes it from
+    tree of kernel types, every hyperparameter's bounds ("fixed" removes it from
     theta), the length-scale shapes and Matern's nu."""
     if isinstance(kernel, (kernels.Sum, kernels.Product)):
         return (type(kernel), kernel_structure(kernel.k1), kernel_structure(kernel.k2))
