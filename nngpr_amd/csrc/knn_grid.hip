@@ -794,9 +794,11 @@ __attribute__((amdgpu_waves_per_eu(K <= 16 ? 3 : (K <= 32 ? 2 : 1))))
 void lane_knn_kernel(SearchArgs a) {
     constexpr int B = K == 16 ? kCoopSlots : kCoopSlotsWide;
     constexpr bool KEYS = K != 16;  // key-only list order (see above)
-    // refs per step of a lane, their loads in flight together (4: 2.14 ms, 8: 1.97 ms,
-    // 12: 2.06 ms, 16: 2.62 ms on the 434,874-row Vecchia search)
-    constexpr int CH = 8;
+    // refs per step of a lane, their loads in flight together (with (key, index)
+    // merges 4: 2.14 ms, 8: 1.97 ms, 12: 2.06 ms, 16: 2.62 ms on the 434,874-row
+    // Vecchia search; with the key-only merges of lists of 32 and 64, whole kNN call
+    // 6: 2.08-2.10 ms, 8: 2.02-2.04, 12: 1.97-2.00); B - CH is the merge trigger
+    constexpr int CH = K == 16 ? 8 : 12;
     __shared__ double sk[kLaneWaves][B][64];
     __shared__ int si[kLaneWaves][B][64];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
@@ -997,10 +999,11 @@ void lane_knn_kernel(SearchArgs a) {
                 const int nb = wave_max32(cnt);
                 // K = 16: insertion ladders (the networks would not fit its 3-wave
                 // register budget); K = 32 and 64: slices of 8 slots through one
-                // network instance (24 slots, merged when a lane holds more than 16:
-                // 1.99 -> 1.91 ms at K = 32 and 6.14 -> 5.83 ms at K = 64 on the
-                // 434,874-row Vecchia search, against 16 slots merged above 8; a
-                // 16-slot network beside the 8-slot one spills)
+                // network instance (24 slots, merged when a lane holds more than
+                // B - CH = 12; with (key, index) merges and CH = 8, 24 slots merged
+                // above 16 took 1.91 ms at K = 32 and 5.83 ms at K = 64 on the
+                // 434,874-row Vecchia search against 1.99 and 6.14 ms for 16 slots
+                // merged above 8; a 16-slot network beside the 8-slot one spills)
                 if (K == 16) {
 #pragma unroll 1
                     for (int b = 0; b < nb; ++b) {
